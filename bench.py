@@ -1,0 +1,222 @@
+"""Benchmark: fillsinks / excess topography on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fill|fsm]
+
+A step is one call of the hot path over one synthetic 8192 x 8192 float32
+DEM already resident in HBM:
+
+* fill (default): fillsinks on a fractal DEM (Hurst 0.7, 0-3000 m) with 200
+  paraboloid depressions (30% flat-bottomed) and a 2% NaN blob on an edge,
+  NaN cells handled by the reference caller convention (bc = 1, dem = -INF).
+* fsm: excesstopography_fsm2d on a fractal DEM (Hurst 0.8, 0-1000 m, 5%
+  pits), cellsize 30 m, threshold slopes from 64 Voronoi cells U(0.2, 0.8).
+
+Prints one JSON line. `value` is Gcells/s of the selected workload;
+`roofline` prices the persistent kernel's launch time against the
+algorithmic bytes (fill 9 B/cell: dem + bc + output; fsm 12 B/cell: dem +
+slopes + output) and 8 TB/s; `cpu_baseline` times the reference path on
+the host (one core) on a bounded crop of the same DEM. The other workload
+is measured too and reported under "extra".
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = ("fillsinks/excess-topo Gcells/s & %HBM peak, 8192² fp32; "
+          "fwd+adjoint vs plain")
+HBM_PEAK_GBS = 8000.0
+N = 8192
+BYTES_PER_CELL = {"fill": 9, "fsm": 12}
+
+
+def make_inputs(workload: str, n: int):
+    from libtopotoolbox_amd import synthetic as S
+
+    if workload == "fill":
+        dem = S.dem_depressions(n, n, 2)
+        dem, bc, nanm = S.nan_to_bc(dem, S.edge_bc(n, n))
+        return {"dem": dem, "bc": bc}
+    dem = S.dem_pits(n, n, 1)
+    try:
+        import torch
+        dev = "cuda" if torch.cuda.is_available() else None
+    except ImportError:
+        dev = None
+    slopes = S.voronoi_slopes(n, n, 64, 0.2, 0.8, 33, device=dev)
+    return {"dem": dem, "slopes": slopes}
+
+
+def gpu_run(workload: str, arrays: dict, steps: int, warmup: int):
+    import torch
+
+    import libtopotoolbox_amd as tt
+
+    dev = torch.device("cuda:0")
+    dem = torch.from_numpy(arrays["dem"]).to(dev)
+    out = torch.empty_like(dem)
+    if workload == "fill":
+        bc = torch.from_numpy(arrays["bc"]).to(dev)
+
+        def step(stats=False):
+            return tt.fillsinks(dem, bc, out=out, stats=stats)
+    else:
+        slopes = torch.from_numpy(arrays["slopes"]).to(dev)
+
+        def step(stats=False):
+            return tt.excesstopography_fsm2d(dem, slopes, 30.0, out=out, stats=stats)
+
+    _, st = step(stats=True)  # also a warmup; reports rounds / visits
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch_ms = [a.elapsed_time(b) for a, b in ev]
+    return {"wall_s": wall, "launch_ms": per_launch_ms, "stats": st, "out": out}
+
+
+def roofline(workload: str, cells: int, launch_ms: float, traffic):
+    algo = BYTES_PER_CELL[workload] * cells
+    achieved = algo / (launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (profiles/traffic.json), or None when none matches this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(workload, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(workload: str, arrays: dict, crop: int):
+    """Reference path on one host core over the top-left crop x crop cells."""
+    import oracle
+
+    sub = {k: np.ascontiguousarray(v[:crop, :crop]) for k, v in arrays.items()}
+    kind = "reference" if oracle.ref() is not None else "port"
+    cells = crop * crop
+    if workload == "fill":
+        from libtopotoolbox_amd.synthetic import edge_bc
+
+        bc = sub["bc"] | edge_bc(crop, crop)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_fillsinks(sub["dem"], bc)
+        else:
+            oracle.fillsinks(sub["dem"], bc)
+        dt = time.perf_counter() - t0
+        sample = f"fillsinks on the top-left {crop}x{crop} crop (edge bc), one call"
+    else:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_fsm2d(sub["dem"], sub["slopes"], 30.0)
+        else:
+            oracle.fsm2d_f32(sub["dem"], sub["slopes"], 30.0)
+        dt = time.perf_counter() - t0
+        sample = f"excesstopography_fsm2d on the top-left {crop}x{crop} crop, one call"
+    return {"value": round(cells / dt / 1e9, 5), "unit": "Gcells/s", "cores": 1, "kind": kind,
+            "sample": sample, "seconds": round(dt, 2),
+            "host": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["fill", "fsm"], default="fill")
+    ap.add_argument("--size", type=int, default=N)
+    ap.add_argument("--cpu-crop", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    n = args.size
+    cells = n * n
+    other = "fsm" if args.workload == "fill" else "fill"
+
+    t0 = time.perf_counter()
+    arrays = make_inputs(args.workload, n)
+    print(f"[bench] inputs for {args.workload} {n}x{n} in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    r = gpu_run(args.workload, arrays, args.steps, args.warmup)
+    ms = r["wall_s"] * 1e3 / args.steps
+    launch_ms = float(np.mean(r["launch_ms"]))
+    print(f"[bench] {args.workload}: {ms:.3f} ms/step, stats {r['stats']}", file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": METRIC,
+        "value": round(cells / (ms * 1e-3) / 1e9, 4),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": ("fillsinks" if args.workload == "fill"
+                                else "excesstopography_fsm2d") + f" {n}x{n} float32",
+                   "generator": ("fractal H0.7 0-3000 m + 200 paraboloid depressions + "
+                                 "2% NaN blob (bc=1, dem=-INF)" if args.workload == "fill"
+                                 else "fractal H0.8 0-1000 m + 5% pits; 64 Voronoi slopes "
+                                 "U(0.2,0.8); cellsize 30 m"),
+                   "parallelism": "single GPU", "tile": [r["stats"]["tile_x"],
+                                                         r["stats"]["tile_y"]]},
+        "roofline": roofline(args.workload, cells, launch_ms,
+                             measured_traffic(args.workload)),
+        "kernel_stats": {k: r["stats"][k] for k in ("rounds", "visits", "local_iters",
+                                                    "max_local", "grid")},
+    }
+    del r
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload, arrays, min(args.cpu_crop, n))
+    del arrays
+    if not args.no_extra:
+        a2 = make_inputs(other, n)
+        r2 = gpu_run(other, a2, max(3, args.steps // 2), 1)
+        ms2 = r2["wall_s"] * 1e3 / max(3, args.steps // 2)
+        ex = {"workload": other, "ms_per_step": round(ms2, 4),
+              "gcells_per_s": round(cells / (ms2 * 1e-3) / 1e9, 4),
+              "roofline": roofline(other, cells, float(np.mean(r2["launch_ms"])),
+                                   measured_traffic(other)),
+              "kernel_stats": {k: r2["stats"][k] for k in ("rounds", "visits", "local_iters",
+                                                           "max_local", "grid")}}
+        if not args.no_cpu:
+            ex["cpu_baseline"] = cpu_baseline(other, a2, min(2048, n))
+        line["extra"] = {other: ex}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

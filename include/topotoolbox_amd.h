@@ -1,0 +1,105 @@
+/*
+  topotoolbox_amd.h - C ABI of the MI355X (gfx950) sink-filling and
+  excess-topography library.
+
+  Part 1 declares the reference's own entry points with the reference's
+  exact signatures, so a C caller of libtopotoolbox links against
+  libtopotoolbox_amd.so unchanged. They take host pointers, copy to the
+  GPU, run the HIP kernels and copy back before returning.
+
+  Part 2 adds device-pointer entry points (stream-ordered, for ctypes /
+  torch callers whose data already lives in HBM), the spill-index forward
+  and the adjoints, the batched tile entry, and error / tuning hooks. None
+  of these have a counterpart in the reference; each says what it is for.
+
+  Layout everywhere: cell (i, j) is at p = j * dims[0] + i, dims[0]
+  changing fastest (reference include/topotoolbox.h:284-290). In the
+  device entry points nx = dims[0] and ny = dims[1].
+
+  Errors: the reference functions return void and "always succeed"
+  (reference docs/ARCHITECTURE.md:83-98). Here a failure (no GPU, out of
+  memory, a kernel error) leaves the outputs unspecified, is reported on
+  stderr once, and is readable from tt_last_error(). There is no CPU
+  fallback.
+*/
+#ifndef TOPOTOOLBOX_AMD_H
+#define TOPOTOOLBOX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOPOTOOLBOX_AMD_VERSION_MAJOR 0
+#define TOPOTOOLBOX_AMD_VERSION_MINOR 1
+
+/* ---------------------------------------------------------------------
+   Part 1: drop-in replacements (host pointers)
+   --------------------------------------------------------------------- */
+
+/* Replaces has_topotoolbox, reference include/topotoolbox.h:47-48
+   (src/topotoolbox.c:4-5). Always returns 1. */
+int has_topotoolbox(void);
+
+/* Replaces fillsinks, reference include/topotoolbox.h:292-293
+   (src/fillsinks.c:22-48). `dem` is read only (the reference mutates and
+   restores it; the observable contract is the same). */
+void fillsinks(float *output, float *dem, uint8_t *bc, ptrdiff_t dims[2]);
+
+/* Replaces reconstruct, reference include/topotoolbox.h:1055-1056
+   (src/reconstruct.c:270-279). Runs to convergence; the reference stops
+   after 1000 iterations, which matters only for inputs that need more. */
+void reconstruct(float *marker, float *mask, ptrdiff_t dims[2]);
+
+/* Replaces excesstopography_fsm2d, reference
+   include/topotoolbox.h:607-609 (src/excesstopography.c:47-111). */
+void excesstopography_fsm2d(float *excess, float *dem, float *threshold_slopes,
+                            float cellsize, ptrdiff_t dims[2]);
+
+/* ---------------------------------------------------------------------
+   Part 2: device-pointer API (no reference counterpart)
+   All pointers are device pointers; `stream` is a hipStream_t (NULL = the
+   default stream). Calls are stream-ordered and asynchronous unless
+   `stats` is non-NULL, in which case the call waits for completion and
+   fills *stats. Return 0 on success, a nonzero code otherwise
+   (message in tt_last_error()).
+   --------------------------------------------------------------------- */
+
+typedef struct tt_stats {
+  int64_t rounds;      /* persistent-grid rounds until no tile changed */
+  int64_t visits;      /* tile visits summed over rounds */
+  int64_t local_iters; /* local sweep iterations summed over visits */
+  int64_t max_local;   /* largest local iteration count of one visit */
+  int64_t grid;        /* workgroups of the persistent launch */
+  int64_t tile_x;      /* tile width (cells along dims[0]) */
+  int64_t tile_y;      /* tile height (cells along dims[1]) */
+  uint32_t error;      /* device error bits (0 = none) */
+} tt_stats;
+
+/* fillsinks on device buffers (for callers whose DEM is already in HBM). */
+int tt_fillsinks_dev(float *output, const float *dem, const uint8_t *bc, ptrdiff_t nx,
+                     ptrdiff_t ny, void *stream, tt_stats *stats);
+
+/* reconstruct on device buffers; `marker` is updated in place. */
+int tt_reconstruct_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t ny,
+                       void *stream, tt_stats *stats);
+
+/* excesstopography_fsm2d on device buffers. */
+int tt_excesstopography_fsm2d_dev(float *excess, const float *dem, const float *slopes,
+                                  float cellsize, ptrdiff_t nx, ptrdiff_t ny, void *stream,
+                                  tt_stats *stats);
+
+/* Tile shape used by subsequent calls on this thread, in 64x64 quadrants
+   (qx, qy in {1, 2}); returns nonzero for an unsupported shape. */
+int tt_set_tile_shape(int qx, int qy);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *tt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPOTOOLBOX_AMD_H */

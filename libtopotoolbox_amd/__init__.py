@@ -1,0 +1,32 @@
+"""libtopotoolbox_amd: MI355X-native sink filling and excess topography.
+
+Drop-in for libtopotoolbox's fillsinks / reconstruct / excesstopography_fsm2d
+hot path. The compute runs in HIP kernels for gfx950 behind a C ABI
+(include/topotoolbox_amd.h); this package is the host-side mirror of that
+interface for torch tensors (ops), the differentiable wrappers (autograd)
+and the batched tile scheduler (batched).
+"""
+from . import _lib
+from .ops import (
+    excesstopography_fsm2d,
+    excesstopography_fsm2d_host,
+    fillsinks,
+    fillsinks_host,
+    reconstruct,
+    reconstruct_host,
+    set_tile_shape,
+)
+
+__all__ = [
+    "excesstopography_fsm2d",
+    "excesstopography_fsm2d_host",
+    "fillsinks",
+    "fillsinks_host",
+    "reconstruct",
+    "reconstruct_host",
+    "set_tile_shape",
+]
+
+
+def has_topotoolbox() -> int:
+    return _lib.lib().has_topotoolbox()

@@ -1,0 +1,108 @@
+"""ctypes binding of libtopotoolbox_amd.so (the C ABI in include/topotoolbox_amd.h).
+
+The shared library is built in-tree by ``libtopotoolbox_amd.build``. There is
+no fallback: when the library is missing or cannot be loaded, every entry
+point raises ``LibraryNotBuilt``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .build import LIB_PATH
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class TTError(RuntimeError):
+    """A nonzero status from the C ABI."""
+
+
+class Stats(ctypes.Structure):
+    """Mirror of tt_stats."""
+
+    _fields_ = [
+        ("rounds", ctypes.c_int64),
+        ("visits", ctypes.c_int64),
+        ("local_iters", ctypes.c_int64),
+        ("max_local", ctypes.c_int64),
+        ("grid", ctypes.c_int64),
+        ("tile_x", ctypes.c_int64),
+        ("tile_y", ctypes.c_int64),
+        ("error", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+_pd = ctypes.c_ssize_t  # ptrdiff_t
+_dims_t = _pd * 2
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "has_topotoolbox": (ctypes.c_int, []),
+    "fillsinks": (None, [_vp, _vp, _vp, _dims_t]),
+    "reconstruct": (None, [_vp, _vp, _dims_t]),
+    "excesstopography_fsm2d": (None, [_vp, _vp, _vp, ctypes.c_float, _dims_t]),
+    "tt_fillsinks_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_reconstruct_dev": (ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_excesstopography_fsm2d_dev": (
+        ctypes.c_int, [_vp, _vp, _vp, ctypes.c_float, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_fillsinks_spill_dev": (
+        ctypes.c_int, [_vp, _vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_fillsinks_backward_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp]),
+    "tt_excesstopography_fsm2d_backward_dev": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_batch_run_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp,
+                                        ctypes.POINTER(Stats)]),
+    "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "tt_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library (raises LibraryNotBuilt if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryNotBuilt(
+                f"{LIB_PATH} is missing: build it with `python -m libtopotoolbox_amd.build`")
+        try:
+            L = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise LibraryNotBuilt(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list:
+    """Names of the SIGNATURES entries the loaded library exports."""
+    L = lib()
+    return [n for n in SIGNATURES if getattr(L, n, None) is not None]
+
+
+def last_error() -> str:
+    msg = lib().tt_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise TTError(f"{what} failed (status {status}): {last_error()}")
+
+
+def dims(nx: int, ny: int):
+    return _dims_t(nx, ny)

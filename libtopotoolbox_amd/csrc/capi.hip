@@ -1,0 +1,213 @@
+// C ABI: the reference's drop-in entry points (host pointers) and the
+// device-pointer entry points used by the Python layer.
+#include <stdio.h>
+
+#include "../../include/topotoolbox_amd.h"
+#include "kernels.h"
+
+namespace tt {
+const char* last_error();
+}
+
+namespace {
+
+thread_local tt::TileShape g_shape;
+
+void to_public(const tt::RunStats& s, tt_stats* o) {
+  o->rounds = s.rounds;
+  o->visits = s.visits;
+  o->local_iters = s.local_iters;
+  o->max_local = s.max_local;
+  o->grid = s.grid;
+  o->tile_x = s.tile_x;
+  o->tile_y = s.tile_y;
+  o->error = s.error;
+}
+
+int fail(const char* where, hipError_t e) {
+  tt::set_error("%s: %s", where, hipGetErrorString(e));
+  return (int)e;
+}
+
+int device_error(const char* where, uint32_t bits) {
+  tt::set_error("%s: device error bits 0x%x (1 barrier timeout, 2 round cap, 4 local cap)", where,
+                bits);
+  return 1000 + (int)bits;
+}
+
+// Drop-in failures are reported once on stderr: the reference API has no
+// error channel, and a silent failure would hand back garbage.
+void report(const char* fn) {
+  static thread_local int reported = 0;
+  if (!reported) {
+    fprintf(stderr, "libtopotoolbox_amd: %s failed: %s\n", fn, tt::last_error());
+    reported = 1;
+  }
+}
+
+// Host staging for the drop-in entry points.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+};
+
+bool dims_ok(const ptrdiff_t* dims, const char* fn) {
+  if (!dims || dims[0] < 0 || dims[1] < 0) {
+    tt::set_error("%s: invalid dims", fn);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int has_topotoolbox(void) { return 1; }
+
+const char* tt_last_error(void) { return tt::last_error(); }
+
+int tt_set_tile_shape(int qx, int qy) {
+  tt::TileShape s{qx, qy};
+  if (!tt::tile_shape_ok(s)) {
+    tt::set_error("tt_set_tile_shape: unsupported shape %dx%d", qx, qy);
+    return 1;
+  }
+  g_shape = s;
+  return 0;
+}
+
+int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff_t nx,
+                     ptrdiff_t ny, void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_fillsinks_dev: invalid dims");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fill(output, dem, bc, nx, ny, tt::kModeFill, g_shape,
+                              (hipStream_t)stream, stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_fillsinks_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_fillsinks_dev", st.error);
+  }
+  return 0;
+}
+
+int tt_reconstruct_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t ny,
+                       void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_reconstruct_dev: invalid dims");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fill(marker, mask, nullptr, nx, ny, tt::kModeRecon, g_shape,
+                              (hipStream_t)stream, stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_reconstruct_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_reconstruct_dev", st.error);
+  }
+  return 0;
+}
+
+int tt_excesstopography_fsm2d_dev(float* excess, const float* dem, const float* slopes,
+                                  float cellsize, ptrdiff_t nx, ptrdiff_t ny, void* stream,
+                                  tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_excesstopography_fsm2d_dev: invalid dims");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fsm(excess, dem, slopes, cellsize, nx, ny, g_shape, (hipStream_t)stream,
+                             stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_excesstopography_fsm2d_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_excesstopography_fsm2d_dev", st.error);
+  }
+  return 0;
+}
+
+void fillsinks(float* output, float* dem, uint8_t* bc, ptrdiff_t dims[2]) {
+  if (!dims_ok(dims, "fillsinks")) return report("fillsinks");
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return;
+  DevBuf d_out, d_dem, d_bc;
+  hipError_t e = d_out.alloc(n * 4);
+  if (e == hipSuccess) e = d_dem.alloc(n * 4);
+  if (e == hipSuccess) e = d_bc.alloc(n);
+  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_bc.p, bc, n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail("fillsinks", e);
+    return report("fillsinks");
+  }
+  tt_stats st;
+  if (tt_fillsinks_dev((float*)d_out.p, (const float*)d_dem.p, (const uint8_t*)d_bc.p, dims[0],
+                       dims[1], nullptr, &st) != 0)
+    return report("fillsinks");
+  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail("fillsinks", e);
+    report("fillsinks");
+  }
+}
+
+void reconstruct(float* marker, float* mask, ptrdiff_t dims[2]) {
+  if (!dims_ok(dims, "reconstruct")) return report("reconstruct");
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return;
+  DevBuf d_marker, d_mask;
+  hipError_t e = d_marker.alloc(n * 4);
+  if (e == hipSuccess) e = d_mask.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_marker.p, marker, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_mask.p, mask, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail("reconstruct", e);
+    return report("reconstruct");
+  }
+  tt_stats st;
+  if (tt_reconstruct_dev((float*)d_marker.p, (const float*)d_mask.p, dims[0], dims[1], nullptr,
+                         &st) != 0)
+    return report("reconstruct");
+  e = hipMemcpy(marker, d_marker.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail("reconstruct", e);
+    report("reconstruct");
+  }
+}
+
+void excesstopography_fsm2d(float* excess, float* dem, float* threshold_slopes, float cellsize,
+                            ptrdiff_t dims[2]) {
+  if (!dims_ok(dims, "excesstopography_fsm2d")) return report("excesstopography_fsm2d");
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return;
+  DevBuf d_u, d_dem, d_s;
+  hipError_t e = d_u.alloc(n * 4);
+  if (e == hipSuccess) e = d_dem.alloc(n * 4);
+  if (e == hipSuccess) e = d_s.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_s.p, threshold_slopes, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail("excesstopography_fsm2d", e);
+    return report("excesstopography_fsm2d");
+  }
+  tt_stats st;
+  if (tt_excesstopography_fsm2d_dev((float*)d_u.p, (const float*)d_dem.p, (const float*)d_s.p,
+                                    cellsize, dims[0], dims[1], nullptr, &st) != 0)
+    return report("excesstopography_fsm2d");
+  e = hipMemcpy(excess, d_u.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail("excesstopography_fsm2d", e);
+    report("excesstopography_fsm2d");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Host-side interface of the HIP kernels (C++; the C ABI lives in capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tt {
+
+struct RunStats {
+  int64_t rounds = 0;       // persistent-grid rounds
+  int64_t visits = 0;       // tile visits
+  int64_t local_iters = 0;  // local sweep iterations summed over visits
+  int64_t max_local = 0;    // largest local iteration count of one visit
+  int64_t grid = 0;         // workgroups in the persistent grid
+  int64_t tile_x = 0, tile_y = 0;
+  uint32_t error = 0;       // Sync error bits
+};
+
+// Tile shape in 64x64 quadrants (one wavefront per quadrant).
+struct TileShape {
+  int qx = 2, qy = 2;
+};
+
+enum FillMode : int {
+  kModeFill = 0,   // fillsinks: state = output levels, src = dem, bc used
+  kModeRecon = 1,  // reconstruct: state = marker (in place), src = mask
+};
+
+// Sink filling / reconstruction. `stats` (host) may be null; when set the
+// call synchronises the stream to read the control block.
+hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
+                    int mode, TileShape ts, hipStream_t stream, RunStats* stats);
+
+// Fast-sweeping excess topography: u (output) from dem and slopes.
+hipError_t run_fsm(float* u, const float* dem, const float* slopes, float cellsize, int64_t nx,
+                   int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
+
+// Thread-local error message set by the entry points.
+void set_error(const char* fmt, ...);
+
+// Checks a tile shape is one of the compiled variants.
+bool tile_shape_ok(TileShape ts);
+
+}  // namespace tt

@@ -1,0 +1,82 @@
+// Workspace, occupancy and error plumbing shared by the kernels.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "workspace.h"
+
+namespace tt {
+
+__global__ void zero_words(unsigned* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0u;
+}
+
+hipError_t Workspace::alloc(int ntiles, hipStream_t s) {
+  stream = s;
+  // Sync block (rounded to 64 words) + mark[ntiles] + list[2 * ntiles]
+  const size_t head = 64;
+  words = head + (size_t)ntiles * 3;
+  hipError_t e = hipMallocAsync(&base, words * sizeof(unsigned), s);
+  if (e != hipSuccess) return e;
+  sync = reinterpret_cast<Sync*>(base);
+  mark = reinterpret_cast<unsigned*>(base) + head;
+  list = mark + ntiles;
+  int blocks = (int)((words + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  zero_words<<<blocks, 256, 0, s>>>(reinterpret_cast<unsigned*>(base), words);
+  return hipGetLastError();
+}
+
+hipError_t Workspace::finish(RunStats* st) {
+  hipError_t e = hipSuccess;
+  if (st) {
+    Sync h;
+    e = hipMemcpyAsync(&h, sync, sizeof(Sync), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) {
+      st->rounds = h.rounds;
+      st->visits = (int64_t)h.visits;
+      st->local_iters = (int64_t)h.local_iters;
+      st->max_local = h.max_local;
+      st->error = h.error;
+    }
+  }
+  hipError_t f = hipFreeAsync(base, stream);
+  base = nullptr;
+  return e != hipSuccess ? e : f;
+}
+
+hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) return hipErrorInvalidConfiguration;
+  long long g = (long long)cus * per_cu;
+  if (g > ntiles) g = ntiles;
+  if (g < 1) g = 1;
+  *grid = (int)g;
+  return hipSuccess;
+}
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+bool tile_shape_ok(TileShape ts) {
+  return (ts.qx == 1 || ts.qx == 2) && (ts.qy == 1 || ts.qy == 2);
+}
+
+}  // namespace tt

@@ -1,0 +1,165 @@
+"""Host-side mirror of the reference interface for the hot path.
+
+Two flavours of each reference entry point:
+
+* ``fillsinks`` / ``reconstruct`` / ``excesstopography_fsm2d`` take torch
+  tensors already on the GPU (2-D, float32, C-contiguous, shape (ny, nx) so
+  that dims = (nx, ny) with dims[0] fastest, as in the reference) and run on
+  the current torch stream through the device-pointer C ABI.
+* ``*_host`` take numpy arrays and call the drop-in C symbols with the
+  reference's exact signatures (host pointers, copies in and out).
+
+Argument meaning follows include/topotoolbox.h of the reference:
+fillsinks (:256-293), reconstruct (:1045-1056), excesstopography_fsm2d
+(:536-609).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check_grid(t: torch.Tensor, name: str, dtype=torch.float32) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on the GPU")
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be 2-D (ny, nx)")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be C-contiguous")
+
+
+def _same_shape(ref: torch.Tensor, *others) -> None:
+    for name, t in others:
+        if t.shape != ref.shape:
+            raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(ref.shape)}")
+        if t.device != ref.device:
+            raise ValueError(f"{name} is on {t.device}, expected {ref.device}")
+
+
+def _stats_ptr(want: bool):
+    return _lib.Stats() if want else None
+
+
+def _ret(out, st):
+    return (out, st.as_dict()) if st is not None else out
+
+
+def set_tile_shape(qx: int, qy: int) -> None:
+    """Tile shape (in 64x64 quadrants) for later calls on this thread."""
+    _lib.check(_lib.lib().tt_set_tile_shape(qx, qy), "tt_set_tile_shape")
+
+
+def fillsinks(dem: torch.Tensor, bc: torch.Tensor, out: torch.Tensor | None = None,
+              stats: bool = False):
+    """Filled DEM. ``bc == 1`` marks fixed cells (reference fillsinks)."""
+    _check_grid(dem, "dem")
+    _check_grid(bc, "bc", torch.uint8)
+    _same_shape(dem, ("bc", bc))
+    if out is None:
+        out = torch.empty_like(dem)
+    else:
+        _check_grid(out, "out")
+        _same_shape(dem, ("out", out))
+    ny, nx = dem.shape
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_fillsinks_dev(out.data_ptr(), dem.data_ptr(), bc.data_ptr(), nx, ny,
+                                         _stream_handle(), st)
+    _lib.check(rc, "fillsinks")
+    return _ret(out, st)
+
+
+def reconstruct(marker: torch.Tensor, mask: torch.Tensor, inplace: bool = False,
+                stats: bool = False):
+    """Grayscale reconstruction by dilation of ``mask`` from ``marker``."""
+    _check_grid(marker, "marker")
+    _check_grid(mask, "mask")
+    _same_shape(marker, ("mask", mask))
+    m = marker if inplace else marker.clone()
+    ny, nx = m.shape
+    st = _stats_ptr(stats)
+    with torch.cuda.device(m.device):
+        rc = _lib.lib().tt_reconstruct_dev(m.data_ptr(), mask.data_ptr(), nx, ny,
+                                           _stream_handle(), st)
+    _lib.check(rc, "reconstruct")
+    return _ret(m, st)
+
+
+def excesstopography_fsm2d(dem: torch.Tensor, threshold_slopes: torch.Tensor, cellsize: float,
+                           out: torch.Tensor | None = None, stats: bool = False):
+    """Solution of the constrained eikonal equation (subtract from dem for
+    the excess topography), reference excesstopography_fsm2d."""
+    _check_grid(dem, "dem")
+    _check_grid(threshold_slopes, "threshold_slopes")
+    _same_shape(dem, ("threshold_slopes", threshold_slopes))
+    if out is None:
+        out = torch.empty_like(dem)
+    else:
+        _check_grid(out, "out")
+        _same_shape(dem, ("out", out))
+    ny, nx = dem.shape
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_excesstopography_fsm2d_dev(out.data_ptr(), dem.data_ptr(),
+                                                      threshold_slopes.data_ptr(),
+                                                      float(cellsize), nx, ny,
+                                                      _stream_handle(), st)
+    _lib.check(rc, "excesstopography_fsm2d")
+    return _ret(out, st)
+
+
+# ------------------------------------------------------------------ host drop-ins
+def _host(a, dtype) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.ndim != 2:
+        raise ValueError("arrays must be 2-D (ny, nx)")
+    return a
+
+
+def fillsinks_host(dem: np.ndarray, bc: np.ndarray) -> np.ndarray:
+    """Reference-signature fillsinks on host arrays (copies through the GPU)."""
+    dem = _host(dem, np.float32)
+    bc = _host(bc, np.uint8)
+    out = np.empty_like(dem)
+    ny, nx = dem.shape
+    _lib.lib().tt_last_error()
+    _lib.lib().fillsinks(out.ctypes.data, dem.ctypes.data, bc.ctypes.data, _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out
+
+
+def reconstruct_host(marker: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    m = _host(marker, np.float32).copy()
+    mask = _host(mask, np.float32)
+    ny, nx = m.shape
+    _lib.lib().reconstruct(m.ctypes.data, mask.ctypes.data, _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return m
+
+
+def excesstopography_fsm2d_host(dem: np.ndarray, slopes: np.ndarray,
+                                cellsize: float) -> np.ndarray:
+    dem = _host(dem, np.float32)
+    slopes = _host(slopes, np.float32)
+    out = np.empty_like(dem)
+    ny, nx = dem.shape
+    _lib.lib().excesstopography_fsm2d(out.ctypes.data, dem.ctypes.data, slopes.ctypes.data,
+                                      float(cellsize), _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out

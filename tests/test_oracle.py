@@ -1,0 +1,338 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Which check pins which oracle function:
+
+* oracle_pcg4d                 known answers from the reference's pcg4d
+                               (test/utils.c:6-28), fixture pcg4d_kat.npz
+* oracle_fillsinks /
+  oracle_reconstruct           golden fixtures written by the reference
+                               library; live comparison with oracle/_ref
+                               where it is built; a numpy brute-force fixed
+                               point on tiny grids; closed forms (single pit,
+                               notched bowl, nested bowls); invariants
+* oracle_fillsinks_spill / _pf bitwise against the restatement; spill
+                               cells carry the output value; lexicographic
+                               brute force on tiny grids
+* oracle_fillsinks_adjoint_f64 exact finite differences (outputs are copies
+                               of inputs) and conservation of the sum
+* oracle_fsm2d_f32             golden fixtures (bitwise) and oracle/_ref
+* oracle_fsm2d_f64             reference FMM fixtures within 1e-4 (the
+                               reference's own FSM==FMM test), closed forms
+                               (1-D ramp, plane, single source diagonal)
+* oracle_fmm2d_f64             equals FSM f64 within 1e-4; reference FMM
+* oracle_fsm2d_adjoint_f64     FP64 central finite differences
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise, load_golden
+
+
+# ------------------------------------------------------------------ helpers
+def brute_fill(dem, bc):
+    """Jacobi iteration of f = max(d, min(f, 8 neighbours)) from f0 = dem on
+    bc==1 cells and +INF elsewhere; NaN cells are impassable."""
+    nanm = np.isnan(dem)
+    d = np.where(nanm, np.inf, dem).astype(np.float64)
+    f = np.where((bc == 1) & ~nanm, d, np.inf)
+    ny, nx = dem.shape
+    while True:
+        pad = np.full((ny + 2, nx + 2), np.inf)
+        pad[1:-1, 1:-1] = f
+        m = f.copy()
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                m = np.minimum(m, pad[1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx])
+        g = np.where(nanm, np.inf, np.maximum(d, m))
+        if np.array_equal(g, f):
+            break
+        f = g
+    out = f.astype(np.float32)
+    out[nanm] = dem[nanm]
+    return out
+
+
+def edge_bc(shape):
+    bc = np.zeros(shape, np.uint8)
+    bc[0, :] = bc[-1, :] = 1
+    bc[:, 0] = bc[:, -1] = 1
+    return bc
+
+
+# ------------------------------------------------------------------ pcg4d
+def test_pcg4d_known_answers(oracle_lib):
+    kat = load_golden("pcg4d_kat")
+    for args, val in zip(kat["args"], kat["values"]):
+        got = np.float32(oracle_lib.pcg4d(*[int(a) for a in args]))
+        assert got == val, (args, got, val)
+
+
+# ------------------------------------------------------------------ fillsinks
+@pytest.mark.parametrize("name", ["fillsinks_random_s0", "fillsinks_random_s1",
+                                  "fillsinks_random_s2", "fillsinks_nan_raw",
+                                  "fillsinks_nan_convention", "fillsinks_flats"])
+def test_fillsinks_golden(oracle_lib, name):
+    g = load_golden(name)
+    out, it, conv = oracle_lib.fillsinks(g["dem"], g["bc"])
+    assert conv and it < 1000
+    assert_bitwise(out, g["out"], name)
+    assert_bitwise(oracle_lib.fillsinks_pf(g["dem"], g["bc"]), g["out"], name + " (pf)")
+
+
+def test_reconstruct_golden(oracle_lib):
+    g = load_golden("reconstruct_random")
+    out, it, conv = oracle_lib.reconstruct(g["marker"], g["mask"])
+    assert conv
+    assert_bitwise(out, g["out"], "reconstruct")
+
+
+def test_fillsinks_vs_reference_library(oracle_lib):
+    if oracle_lib.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(3)
+    for seed in range(6):
+        ny, nx = rng.integers(1, 90, size=2)
+        dem = oracle_lib.grid_random(seed, (int(ny), int(nx)))
+        dem[rng.uniform(size=dem.shape) < 0.03] = np.nan
+        bc = (rng.uniform(size=dem.shape) < 0.02).astype(np.uint8)
+        bc |= edge_bc(dem.shape)
+        out, _, conv = oracle_lib.fillsinks(dem, bc)
+        assert conv
+        assert_bitwise(out, oracle_lib.ref_fillsinks(dem, bc), f"seed {seed}")
+        marker = np.where(rng.uniform(size=dem.shape) < 0.2, dem + 3.0, -np.inf)
+        marker = marker.astype(np.float32)
+        r, _, _ = oracle_lib.reconstruct(marker, dem)
+        assert_bitwise(r, oracle_lib.ref_reconstruct(marker, dem), f"recon {seed}")
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_fillsinks_brute_force(oracle_lib, seed):
+    rng = np.random.default_rng(seed)
+    ny, nx = rng.integers(1, 12, size=2)
+    dem = rng.integers(0, 6, size=(ny, nx)).astype(np.float32)  # many ties
+    if seed % 2:
+        dem[rng.uniform(size=dem.shape) < 0.15] = np.nan
+    bc = (rng.uniform(size=dem.shape) < 0.25).astype(np.uint8)
+    want = brute_fill(dem, bc)
+    out, _, conv = oracle_lib.fillsinks(dem, bc)
+    assert conv
+    assert_bitwise(out, want, "restatement")
+    assert_bitwise(oracle_lib.fillsinks_pf(dem, bc), want, "priority-flood")
+
+
+def test_fillsinks_closed_forms(oracle_lib):
+    # single pit in a plane fills to the plane
+    dem = np.full((9, 9), 10.0, np.float32)
+    dem[4, 4] = 5.0
+    out, _, _ = oracle_lib.fillsinks(dem, edge_bc(dem.shape))
+    assert np.all(out == 10.0)
+    # cone-shaped bowl with a notch in the rim fills to the notch height
+    n = 41
+    y, x = np.mgrid[0:n, 0:n]
+    r = np.hypot(x - 20, y - 20)
+    dem = np.where(r < 15, r, 30.0 - r).astype(np.float32)
+    dem[20, 5:] = np.minimum(dem[20, 5:], 12.5)  # notch along a row to the edge
+    out, _, _ = oracle_lib.fillsinks(dem, edge_bc(dem.shape))
+    inside = r < 12
+    assert np.all(out[inside] == np.float32(12.5))
+    # nested square bowls (Chebyshev rings, no diagonal leaks): inner pit,
+    # inner rim 10, middle basin 5, outer rim 20 notched to 15 on one side
+    rc = np.maximum(np.abs(x - 20), np.abs(y - 20))
+    dem2 = np.select([rc < 4, rc < 6, rc < 12, rc < 14], [1.0, 10.0, 5.0, 20.0], 0.0)
+    dem2 = dem2.astype(np.float32)
+    dem2[20, 32:34] = 15.0
+    out2, _, _ = oracle_lib.fillsinks(dem2, edge_bc(dem2.shape))
+    assert np.all(out2[rc < 12] == 15.0)
+    assert np.all(out2[rc >= 14] == 0.0)
+
+
+def test_fillsinks_invariants(oracle_lib):
+    dem = oracle_lib.grid_random(99, (64, 80))
+    bc = edge_bc(dem.shape)
+    out, _, _ = oracle_lib.fillsinks(dem, bc)
+    assert np.all(out >= dem)
+    assert_bitwise(out[bc == 1], dem[bc == 1], "bc cells")
+    again, _, _ = oracle_lib.fillsinks(out, bc)
+    assert_bitwise(again, out, "idempotence")
+    # no interior pit: every interior cell has a neighbour not strictly higher
+    pad = np.pad(out, 1, constant_values=np.inf)
+    higher = np.ones_like(out, bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dy or dx:
+                higher &= pad[1 + dy:1 + dy + out.shape[0], 1 + dx:1 + dx + out.shape[1]] > out
+    assert not higher[1:-1, 1:-1].any()
+    # every value is a copy of some input value
+    assert np.isin(out, dem).all()
+
+
+def test_reconstruct_cap_and_resume(oracle_lib):
+    dem = oracle_lib.grid_random(5, (60, 60))
+    bc = edge_bc(dem.shape)
+    full, it, conv = oracle_lib.fillsinks(dem, bc)
+    assert conv and it > 1
+    part, it1, conv1 = oracle_lib.fillsinks(dem, bc, max_iterations=1)
+    assert it1 == 1 and not conv1
+    assert not np.array_equal(part, full)
+    # the partial state is an upper bound; resuming from it (reconstruct on
+    # the negated arrays) reaches the same fixed point
+    marker = -part
+    res, _, conv2 = oracle_lib.reconstruct(marker, -dem)
+    assert conv2
+    assert_bitwise(-res, full, "resume")
+
+
+# ------------------------------------------------------------------ spill index
+def brute_spill(dem, bc):
+    """Lexicographic (value, index) fixed point on a tiny grid."""
+    ny, nx = dem.shape
+    n = ny * nx
+    nanm = np.isnan(dem).reshape(-1)
+    d = dem.reshape(-1).astype(np.float64)
+    key = [(np.inf, n) for _ in range(n)]
+    for p in range(n):
+        if bc.reshape(-1)[p] == 1 and not nanm[p]:
+            key[p] = (d[p], p)
+    changed = True
+    while changed:
+        changed = False
+        for p in range(n):
+            if nanm[p] or bc.reshape(-1)[p] == 1:
+                continue
+            j, i = divmod(p, nx)
+            best = key[p]
+            for dj in (-1, 0, 1):
+                for di in (-1, 0, 1):
+                    qj, qi = j + dj, i + di
+                    if (dj or di) and 0 <= qj < ny and 0 <= qi < nx:
+                        q = qj * nx + qi
+                        if not nanm[q] and key[q] < best:
+                            best = key[q]
+            new = max((d[p], p), best) if best[0] < np.inf else best
+            if new < key[p]:
+                key[p] = new
+                changed = True
+    return np.array([k[1] if k[0] < np.inf else -1 for k in key]).reshape(ny, nx)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_spill_index(oracle_lib, seed):
+    rng = np.random.default_rng(100 + seed)
+    ny, nx = rng.integers(2, 10, size=2)
+    dem = rng.integers(0, 5, size=(ny, nx)).astype(np.float32)
+    if seed % 3 == 0:
+        dem[rng.uniform(size=dem.shape) < 0.15] = np.nan
+    bc = edge_bc(dem.shape) if seed % 2 else (rng.uniform(size=dem.shape) < 0.3).astype(np.uint8)
+    out, spill = oracle_lib.fillsinks_spill(dem, bc)
+    ok = spill >= 0
+    assert_bitwise(out[ok], dem.reshape(-1)[spill[ok]], "spill carries the value")
+    np.testing.assert_array_equal(spill, brute_spill(dem, bc))
+
+
+def test_fill_adjoint_exact_fd(oracle_lib):
+    rng = np.random.default_rng(8)
+    dem = (rng.uniform(size=(12, 14)) * 100).astype(np.float32)
+    bc = edge_bc(dem.shape)
+    g_out = rng.standard_normal(dem.shape)
+    _, spill = oracle_lib.fillsinks_spill(dem, bc)
+    g_dem = oracle_lib.fillsinks_adjoint_f64(spill, g_out)
+    assert abs(g_dem.sum() - g_out.sum()) < 1e-12 * max(1.0, np.abs(g_out).sum())
+    for p in rng.choice(dem.size, size=20, replace=False):
+        j, i = divmod(int(p), dem.shape[1])
+        up, dn = dem.copy(), dem.copy()
+        up[j, i] = np.float32(dem[j, i] + np.float32(1e-3))
+        dn[j, i] = np.float32(dem[j, i] - np.float32(1e-3))
+        fu = oracle_lib.fillsinks_pf(up, bc).astype(np.float64)
+        fd = oracle_lib.fillsinks_pf(dn, bc).astype(np.float64)
+        fdv = ((g_out * fu).sum() - (g_out * fd).sum()) / (float(up[j, i]) - float(dn[j, i]))
+        assert abs(fdv - g_dem[j, i]) < 1e-6 * max(1.0, abs(g_dem[j, i])), (j, i, fdv, g_dem[j, i])
+
+
+# ------------------------------------------------------------------ excess topography
+@pytest.mark.parametrize("name", ["fsm_random_s0", "fsm_random_s1", "fsm_random_s2",
+                                  "fsm_numerics"])
+def test_fsm_golden(oracle_lib, name):
+    g = load_golden(name)
+    u, it = oracle_lib.fsm2d_f32(g["dem"], g["slopes"], float(g["cellsize"]))
+    assert_bitwise(u, g["fsm"], name)
+    # FP64 FSM and FP64 FMM agree with the reference's FMM within the
+    # reference's own FSM==FMM tolerance (test/excesstopography.cpp:69-82)
+    u64, _ = oracle_lib.fsm2d_f64(g["dem"], g["slopes"], float(g["cellsize"]))
+    m64 = oracle_lib.fmm2d_f64(g["dem"], g["slopes"], float(g["cellsize"]))
+    fin = np.isfinite(g["fmm"])
+    # absolute 1e-4 for the reference's random DEMs (values below 100 m);
+    # the numerics case reaches ~1e3 m, where the reference's float FMM
+    # rounds at ~1e-4 m, so it is held to 1e-6 relative instead
+    scale = 1.0 if name != "fsm_numerics" else 1e-2 * np.maximum(1.0, np.abs(u64[fin]))
+    assert np.all(np.abs(u64[fin] - g["fmm"][fin]) <= 1e-4 * scale)
+    assert np.nanmax(np.abs(m64 - u64)) <= 1e-4
+    np.testing.assert_array_equal(np.isnan(u64), np.isnan(g["fsm"]))
+
+
+def test_fsm_vs_reference_library(oracle_lib):
+    if oracle_lib.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(4)
+    for seed in range(4):
+        ny, nx = rng.integers(1, 70, size=2)
+        dem = (rng.uniform(size=(ny, nx)) * 300).astype(np.float32)
+        dem[rng.uniform(size=dem.shape) < 0.05] = np.nan
+        slopes = rng.uniform(0.05, 1.0, size=dem.shape).astype(np.float32)
+        u, _ = oracle_lib.fsm2d_f32(dem, slopes, 10.0)
+        assert_bitwise(u, oracle_lib.ref_fsm2d(dem, slopes, 10.0), f"seed {seed}")
+
+
+def test_fsm_closed_forms(oracle_lib):
+    big = 1.0e6
+    # 1-D strip: u_i = i * fi (one-sided branch only, exact in double)
+    dem = np.full((1, 50), big)
+    dem[0, 0] = 0.0
+    s = np.full(dem.shape, 0.5)
+    u, _ = oracle_lib.fsm2d_f64(dem, s, 30.0)
+    np.testing.assert_array_equal(u[0], 15.0 * np.arange(50))
+    # plane: zero row, interior high -> u = k * fi on every column
+    dem = np.full((20, 30), big)
+    dem[0, :] = 0.0
+    s = np.full(dem.shape, 0.5)
+    u, _ = oracle_lib.fsm2d_f64(dem, s, 30.0)
+    np.testing.assert_array_equal(u, 15.0 * np.arange(20)[:, None] * np.ones((1, 30)))
+    # single source: axes k*fi, first diagonal fi*(1 + 1/sqrt(2))
+    dem = np.full((10, 10), big)
+    dem[0, 0] = 0.0
+    u, _ = oracle_lib.fsm2d_f64(dem, np.full(dem.shape, 0.5), 30.0)
+    assert u[0, 3] == 45.0 and u[3, 0] == 45.0
+    assert abs(u[1, 1] - 15.0 * (1 + 1 / np.sqrt(2))) < 1e-12
+    # constraint u <= dem and upwind gradient <= slope (+1e-4 on interior)
+    dem = (np.random.default_rng(1).uniform(size=(30, 40)) * 200)
+    s = np.random.default_rng(2).uniform(0.1, 0.9, size=dem.shape)
+    u, _ = oracle_lib.fsm2d_f64(dem, s, 30.0)
+    assert np.all(u <= dem)
+    gx = np.maximum(0, np.maximum((u[:, 1:-1] - u[:, :-2]), (u[:, 1:-1] - u[:, 2:]))) / 30
+    gy = np.maximum(0, np.maximum((u[1:-1, :] - u[:-2, :]), (u[1:-1, :] - u[2:, :]))) / 30
+    g = np.hypot(gx[1:-1, :], gy[:, 1:-1])
+    assert np.all(g - s[1:-1, 1:-1] < 1e-4)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_fsm_adjoint_fd(oracle_lib, seed):
+    rng = np.random.default_rng(200 + seed)
+    ny, nx = 14, 17
+    dem = rng.uniform(0, 300, size=(ny, nx))
+    s = rng.uniform(0.1, 0.9, size=(ny, nx))
+    g_out = rng.standard_normal((ny, nx))
+    cs = 30.0
+    u, _ = oracle_lib.fsm2d_f64(dem, s, cs)
+    g_dem, g_s = oracle_lib.fsm2d_adjoint_f64(u, dem, s, cs, g_out)
+    eps = 1e-6
+
+    def L(d, sl):
+        return (g_out * oracle_lib.fsm2d_f64(d, sl, cs)[0]).sum()
+
+    for p in rng.choice(ny * nx, size=25, replace=False):
+        j, i = divmod(int(p), nx)
+        e = np.zeros((ny, nx))
+        e[j, i] = eps
+        fd_d = (L(dem + e, s) - L(dem - e, s)) / (2 * eps)
+        fd_s = (L(dem, s + e) - L(dem, s - e)) / (2 * eps)
+        assert abs(fd_d - g_dem[j, i]) < 1e-5 * max(1.0, abs(fd_d)), ("dem", j, i, fd_d, g_dem[j, i])
+        assert abs(fd_s - g_s[j, i]) < 1e-5 * max(1.0, abs(fd_s)), ("slope", j, i, fd_s, g_s[j, i])
