@@ -76,6 +76,11 @@ typedef struct tt_stats {
   int64_t tile_x;      /* tile width (cells along dims[0]) */
   int64_t tile_y;      /* tile height (cells along dims[1]) */
   uint32_t error;      /* device error bits (0 = none) */
+  /* wall-clock ticks (100 MHz) summed over workgroups, by phase */
+  int64_t t_stage;     /* staging tiles into LDS */
+  int64_t t_relax;     /* local sweeps */
+  int64_t t_write;     /* write-back and neighbour queueing */
+  int64_t t_wait;      /* grid-barrier waits */
 } tt_stats;
 
 /* fillsinks on device buffers (for callers whose DEM is already in HBM). */
@@ -90,6 +95,28 @@ int tt_reconstruct_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t
 int tt_excesstopography_fsm2d_dev(float *excess, const float *dem, const float *slopes,
                                   float cellsize, ptrdiff_t nx, ptrdiff_t ny, void *stream,
                                   tt_stats *stats);
+
+/* fillsinks that also returns, for every cell, the index of the cell whose
+   DEM value the output copies (its spill cell; -1 for NaN and unreached
+   cells). Ties are broken by the lexicographic (level, index) order, so the
+   index is unique and deterministic. For the adjoint; nx * ny must not
+   exceed INT32_MAX. */
+int tt_fillsinks_spill_dev(float *output, int32_t *spill, const float *dem, const uint8_t *bc,
+                           ptrdiff_t nx, ptrdiff_t ny, void *stream, tt_stats *stats);
+
+/* Adjoint of fillsinks: g_dem[q] = sum of g_out[p] over p with
+   spill[p] == q, accumulated in 64-bit fixed point (resolution 2^-32,
+   |sum| < 2^31), so the result does not depend on the order of the atomic
+   adds. Non-finite g_out entries contribute nothing. */
+int tt_fillsinks_backward_dev(float *g_dem, const int32_t *spill, const float *g_out,
+                              ptrdiff_t nx, ptrdiff_t ny, void *stream);
+
+/* Adjoint of excesstopography_fsm2d at its converged output `excess`:
+   for L = sum(g_out * excess), writes dL/ddem and dL/dthreshold_slopes. */
+int tt_excesstopography_fsm2d_backward_dev(float *g_dem, float *g_slopes, const float *excess,
+                                           const float *dem, const float *slopes, float cellsize,
+                                           const float *g_out, ptrdiff_t nx, ptrdiff_t ny,
+                                           void *stream, tt_stats *stats);
 
 /* Tile shape used by subsequent calls on this thread, in 64x64 quadrants
    (qx, qy in {1, 2}); returns nonzero for an unsupported shape. */

@@ -32,6 +32,10 @@ class Stats(ctypes.Structure):
         ("tile_x", ctypes.c_int64),
         ("tile_y", ctypes.c_int64),
         ("error", ctypes.c_uint32),
+        ("t_stage", ctypes.c_int64),
+        ("t_relax", ctypes.c_int64),
+        ("t_write", ctypes.c_int64),
+        ("t_wait", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:

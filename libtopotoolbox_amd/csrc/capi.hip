@@ -1,5 +1,6 @@
 // C ABI: the reference's drop-in entry points (host pointers) and the
 // device-pointer entry points used by the Python layer.
+#include <limits.h>
 #include <stdio.h>
 
 #include "../../include/topotoolbox_amd.h"
@@ -22,6 +23,10 @@ void to_public(const tt::RunStats& s, tt_stats* o) {
   o->tile_x = s.tile_x;
   o->tile_y = s.tile_y;
   o->error = s.error;
+  o->t_stage = s.t_stage;
+  o->t_relax = s.t_relax;
+  o->t_write = s.t_write;
+  o->t_wait = s.t_wait;
 }
 
 int fail(const char* where, hipError_t e) {
@@ -131,6 +136,56 @@ int tt_excesstopography_fsm2d_dev(float* excess, const float* dem, const float* 
   if (stats) {
     to_public(st, stats);
     if (st.error) return device_error("tt_excesstopography_fsm2d_dev", st.error);
+  }
+  return 0;
+}
+
+int tt_fillsinks_spill_dev(float* output, int32_t* spill, const float* dem, const uint8_t* bc,
+                           ptrdiff_t nx, ptrdiff_t ny, void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || (nx > 0 && ny > INT32_MAX / nx)) {
+    tt::set_error("tt_fillsinks_spill_dev: dims must be >= 0 with nx * ny <= INT32_MAX");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fill_spill(output, spill, dem, bc, nx, ny, g_shape, (hipStream_t)stream,
+                                    stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_fillsinks_spill_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_fillsinks_spill_dev", st.error);
+  }
+  return 0;
+}
+
+int tt_fillsinks_backward_dev(float* g_dem, const int32_t* spill, const float* g_out,
+                              ptrdiff_t nx, ptrdiff_t ny, void* stream) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || (nx > 0 && ny > INT32_MAX / nx)) {
+    tt::set_error("tt_fillsinks_backward_dev: dims must be >= 0 with nx * ny <= INT32_MAX");
+    return 1;
+  }
+  hipError_t e = tt::run_fill_backward(g_dem, spill, g_out, nx, ny, (hipStream_t)stream, nullptr);
+  if (e != hipSuccess) return fail("tt_fillsinks_backward_dev", e);
+  return 0;
+}
+
+int tt_excesstopography_fsm2d_backward_dev(float* g_dem, float* g_slopes, const float* excess,
+                                           const float* dem, const float* slopes, float cellsize,
+                                           const float* g_out, ptrdiff_t nx, ptrdiff_t ny,
+                                           void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_excesstopography_fsm2d_backward_dev: invalid dims");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fsm_backward(g_dem, g_slopes, excess, dem, slopes, cellsize, g_out, nx,
+                                      ny, (hipStream_t)stream, stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_excesstopography_fsm2d_backward_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_excesstopography_fsm2d_backward_dev", st.error);
   }
   return 0;
 }

@@ -7,24 +7,37 @@
 // negates back. That fixed point is unique and every value is a copy of an
 // input value, so any update order reaches the same bits.
 //
-// This kernel works directly in the "fill domain" f = -marker, d = -mask:
+// The kernels work directly in the "fill domain" f = -marker, d = -mask:
 //     f_p <- max(d_p, min(f_p, f_q for the 8 neighbours q))
 // iterated from f0 = d on seeds (+INF elsewhere) down to the greatest fixed
-// point. NaN cells of d are inert: they hold +INF (so fminf ignores them as
-// fmaxf ignores NaN in the reference) and are written back as the input NaN.
+// point. NaN cells of d are inert: level +INF, bound +INF (fminf skips them
+// as fmaxf skips NaN in the reference), written back as the input NaN.
 //
-// fillsinks: d = dem, f0 = (bc == 1 && dem is not NaN) ? dem : +INF; the
-//            output array holds f between rounds.
+// fillsinks:   d = dem, f0 = (bc == 1) ? dem : +INF; the output array
+//              holds f between rounds.
 // reconstruct: d = -mask, f0 = max(d, -marker) (a NaN marker acts as -INF);
-//            the marker array holds -f between rounds and at the end.
+//              the marker array holds -f between rounds and at the end.
+// spill:       fillsinks over lexicographic (level, cell index) keys packed
+//              in 64 bits; the index half of the fixed point is the cell
+//              whose DEM value the output copies (the spill cell), which
+//              the adjoint scatters to. Unique fixed point, so deterministic.
+//              Halo reads race with neighbour write-backs within a round,
+//              which is harmless only because every read value is a valid
+//              upper bound: hence one 64-bit word per key, never two arrays.
+#include <type_traits>
+
 #include "kernels.h"
 #include "tile_engine.h"
 #include "workspace.h"
 
 namespace tt {
 
+enum : int { kModeSpill = 2 };
+
 struct FillArgs {
   float* state;
+  int* spill;                // kModeSpill only
+  unsigned long long* keys;  // kModeSpill only: the iteration state
   const float* src;
   const uint8_t* bc;
   long long nx, ny;
@@ -36,62 +49,98 @@ struct FillArgs {
   unsigned max_local;
 };
 
+// ---------------------------------------------------------------- level types
+struct FloatLevel {
+  using T = float;
+  static __device__ __forceinline__ T lo(T a, T b) { return fminf(a, b); }
+  static __device__ __forceinline__ T hi(T a, T b) { return fmaxf(a, b); }
+  static __device__ __forceinline__ T from_below(T v, T e) { return lane_from_below(v, e); }
+  static __device__ __forceinline__ T from_above(T v, T e) { return lane_from_above(v, e); }
+  static constexpr T kUnreached = __builtin_inff();
+  static constexpr T kInert = __builtin_inff();
+};
+
+// Order-preserving map of a float's bits to uint32 (x < y <=> ord(x) < ord(y)
+// for non-NaN x, y, with -0 ordered below +0).
+__device__ __forceinline__ unsigned ord_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u);
+}
+__device__ __forceinline__ float unord_bits(unsigned u) {
+  return __uint_as_float((u >> 31) ? (u ^ 0x80000000u) : ~u);
+}
+
+__device__ __forceinline__ unsigned dpp_u32(unsigned old, unsigned v, int ctrl) {
+  return ctrl == 0x138
+             ? (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false)
+             : (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+struct KeyLevel {
+  using T = unsigned long long;  // ord(level) << 32 | cell index
+  static __device__ __forceinline__ T lo(T a, T b) { return a < b ? a : b; }
+  static __device__ __forceinline__ T hi(T a, T b) { return a < b ? b : a; }
+  static __device__ __forceinline__ T shift(T v, T e, int ctrl) {
+    const unsigned l = dpp_u32((unsigned)e, (unsigned)v, ctrl);
+    const unsigned h = dpp_u32((unsigned)(e >> 32), (unsigned)(v >> 32), ctrl);
+    return ((T)h << 32) | l;
+  }
+  static __device__ __forceinline__ T from_below(T v, T e) { return shift(v, e, 0x138); }
+  static __device__ __forceinline__ T from_above(T v, T e) { return shift(v, e, 0x130); }
+  // (+INF, no index): a cell no seed has reached yet; all ones: inert bound
+  static constexpr T kUnreached = (0xff800000ull << 32) | 0xffffffffull;
+  static constexpr T kInert = ~0ull;
+  static __device__ __forceinline__ T make(float f, unsigned idx) {
+    return ((T)ord_bits(f) << 32) | idx;
+  }
+};
+
 // One directional sweep over a 64x64 quadrant with lanes along columns
 // (vertical sweep: down = +y) or along rows (horizontal: right = +x).
 // The value carried between steps is the lane's previous cell; its two
 // side neighbours come from the adjacent lanes by DPP, with the quadrant's
-// outside column/row read from LDS for the edge lanes.
-template <class G, bool VERT, bool FWD>
-__device__ __forceinline__ bool fill_sweep(float* sf, const float* sd, int R0, int C0, int lane) {
+// outside column/row read from LDS for the edge lanes. The next step's LDS
+// operands are fetched before this step's store.
+template <class G, class L, bool VERT, bool FWD>
+__device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::T* sd, int R0,
+                                           int C0, int lane) {
+  using T = typename L::T;
   constexpr int step = FWD ? 1 : -1;
   bool changed = false;
-  if (VERT) {
-    const int C = C0 + lane;
-    const int ec = (lane == 0) ? C - 1 : C + 1;
-    int R = FWD ? R0 : R0 + 63;
-    float pc = sf[G::at(R - step, C)];
+  const int own = VERT ? C0 + lane : R0 + lane;      // lane's column / row
+  const int side = (lane == 0) ? own - 1 : own + 1;  // outside neighbour line
+  const int k0 = VERT ? (FWD ? R0 : R0 + 63) : (FWD ? C0 : C0 + 63);
+  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
+  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
+  T pc = sf[cell(k0 - step)];
+  T fc_n = sf[cell(k0)], dv_n = sd[cell(k0)], e_n = sf[edge_cell(k0 - step)];
+  int k = k0;
 #pragma unroll 4
-    for (int k = 0; k < 64; k++, R += step) {
-      float edge = sf[G::at(R - step, ec)];
-      float fc = sf[G::at(R, C)];
-      float dv = sd[G::at(R, C)];
-      float pl = lane_from_below(pc, edge);
-      float pr = lane_from_above(pc, edge);
-      float m = fminf(fminf(pl, pr), fminf(pc, fc));
-      float nv = fmaxf(dv, m);
-      bool upd = (nv < fc) && (dv == dv);
-      if (upd) sf[G::at(R, C)] = nv;
-      changed |= upd;
-      pc = upd ? nv : fc;
+  for (int s = 0; s < 64; s++, k += step) {
+    const T fc = fc_n, dv = dv_n, edge = e_n;
+    if (s < 63) {
+      fc_n = sf[cell(k + step)];
+      dv_n = sd[cell(k + step)];
+      e_n = sf[edge_cell(k)];
     }
-  } else {
-    const int R = R0 + lane;
-    const int er = (lane == 0) ? R - 1 : R + 1;
-    int C = FWD ? C0 : C0 + 63;
-    float pc = sf[G::at(R, C - step)];
-#pragma unroll 4
-    for (int k = 0; k < 64; k++, C += step) {
-      float edge = sf[G::at(er, C - step)];
-      float fc = sf[G::at(R, C)];
-      float dv = sd[G::at(R, C)];
-      float pu = lane_from_below(pc, edge);
-      float pd = lane_from_above(pc, edge);
-      float m = fminf(fminf(pu, pd), fminf(pc, fc));
-      float nv = fmaxf(dv, m);
-      bool upd = (nv < fc) && (dv == dv);
-      if (upd) sf[G::at(R, C)] = nv;
-      changed |= upd;
-      pc = upd ? nv : fc;
-    }
+    const T a = L::from_below(pc, edge);
+    const T b = L::from_above(pc, edge);
+    const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc, fc)));
+    const bool upd = nv < fc;
+    if (upd) sf[cell(k)] = nv;
+    changed |= upd;
+    pc = L::lo(nv, fc);
   }
   return changed;
 }
 
 template <class G, int MODE>
 __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
-  __shared__ float sf[G::CELLS];
-  __shared__ float sd[G::CELLS];
-  __shared__ float sper[2 * (G::TX + G::TY)];
+  using L = typename std::conditional<MODE == kModeSpill, KeyLevel, FloatLevel>::type;
+  using T = typename L::T;
+  __shared__ T sf[G::CELLS];
+  __shared__ T sd[G::CELLS];
+  __shared__ T sper[2 * (G::TX + G::TY)];
   __shared__ unsigned s_bits;
 
   const int tid = threadIdx.x;
@@ -100,35 +149,58 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
   const int R0 = (wave / G::QX) * 64;
   const int C0 = (wave % G::QX) * 64;
   const int ntiles = a.ntx * a.nty;
-  const unsigned nb = gridDim.x;
 
   auto visit = [&](int t, unsigned round) {
+    const unsigned long long c0 = wall_clock64();
     const int tx = t % a.ntx, ty = t / a.ntx;
     const long long x0 = (long long)tx * G::TX, y0 = (long long)ty * G::TY;
-    // ---- stage tile + halo
-    for (int idx = tid; idx < G::ROWS * (G::TX + 2); idx += G::THREADS) {
-      const int r = idx / (G::TX + 2) - 1;
-      const int c = idx % (G::TX + 2) - 1;
-      const long long gx = x0 + c, gy = y0 + r;
-      float f = kInf, d = __builtin_nanf("");
-      if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
-        const size_t g = (size_t)gy * a.nx + gx;
-        const float raw = a.src[g];
-        if (MODE == kModeFill) {
-          d = raw;
-          if (round == 0)
-            f = (a.bc[g] == 1) ? raw : kInf;
-          else
-            f = a.state[g];
-        } else {
-          d = -raw;
-          const float m = a.state[g];
-          f = fmaxf(d, (m == m) ? -m : kInf);
+    // ---- stage tile + halo: U independent global loads in flight per
+    // thread before the LDS stores
+    constexpr int E = G::ROWS * (G::TX + 2);
+    constexpr int U = 8;
+    for (int base = 0; base < E; base += G::THREADS * U) {
+      T fv[U], dv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const int r = idx / (G::TX + 2) - 1;
+        const int c = idx % (G::TX + 2) - 1;
+        const long long gx = x0 + c, gy = y0 + r;
+        T f = L::kUnreached, d = L::kInert;
+        if (idx < E && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
+          const size_t g = (size_t)gy * a.nx + gx;
+          const float raw = a.src[g];
+          if constexpr (MODE == kModeFill) {
+            if (raw == raw) {
+              d = raw;
+              f = (round == 0) ? ((a.bc[g] == 1) ? raw : kInf) : a.state[g];
+            }
+          } else if constexpr (MODE == kModeRecon) {
+            if (raw == raw) {
+              d = -raw;
+              const float m = a.state[g];
+              f = fmaxf(d, (m == m) ? -m : kInf);
+            }
+          } else {
+            if (raw == raw) {
+              d = KeyLevel::make(raw, (unsigned)g);
+              f = (round == 0) ? ((a.bc[g] == 1) ? d : KeyLevel::kUnreached) : a.keys[g];
+            }
+          }
         }
-        if (d != d) f = kInf;
+        fv[u] = f;
+        dv[u] = d;
       }
-      sf[G::at(r, c)] = f;
-      sd[G::at(r, c)] = d;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < E) {
+          const int r = idx / (G::TX + 2) - 1;
+          const int c = idx % (G::TX + 2) - 1;
+          sf[G::at(r, c)] = fv[u];
+          sd[G::at(r, c)] = dv[u];
+        }
+      }
     }
     if (tid == 0) s_bits = 0u;
     __syncthreads();
@@ -140,26 +212,46 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
       sper[2 * G::TX + k] = sf[G::at(k, 0)];
       sper[2 * G::TX + G::TY + k] = sf[G::at(k, G::TX - 1)];
     }
+    const unsigned long long c1 = wall_clock64();
     // ---- relax to the local fixed point
     unsigned it = 0;
-    int any;
+    int any, any_change = 0;
     do {
-      bool ch = fill_sweep<G, true, true>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, true, false>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, false, true>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, false, false>(sf, sd, R0, C0, lane);
+      bool ch = fill_sweep<G, L, true, true>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, true, false>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, false, true>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, false, false>(sf, sd, R0, C0, lane);
       any = __syncthreads_or(ch ? 1 : 0);
+      any_change |= any;
       it++;
     } while (any && it < a.max_local);
-    // ---- write back
-    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-      const int r = idx / G::TX, c = idx % G::TX;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx < a.nx && gy < a.ny) {
-        const float d = sd[G::at(r, c)];
-        float v = (d != d) ? d : sf[G::at(r, c)];
-        if (MODE == kModeRecon) v = -v;
-        a.state[(size_t)gy * a.nx + gx] = v;
+    const unsigned long long c2 = wall_clock64();
+    // ---- write back: round 0 initialises the whole state; later rounds
+    // skip tiles whose relaxation changed nothing. Inert cells (NaN input)
+    // take the input bits.
+    if (round == 0 || any_change) {
+      for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
+        const int r = idx / G::TX, c = idx % G::TX;
+        const long long gx = x0 + c, gy = y0 + r;
+        if (gx < a.nx && gy < a.ny) {
+          const size_t g = (size_t)gy * a.nx + gx;
+          const T v = sf[G::at(r, c)];
+          const bool inert = sd[G::at(r, c)] == L::kInert;
+          if constexpr (MODE == kModeSpill) {
+            // one aligned 64-bit store: a neighbour reading its halo
+            // concurrently sees the old or the new key, never a mix
+            a.keys[g] = v;
+            a.state[g] = inert ? a.src[g] : unord_bits((unsigned)(v >> 32));
+            a.spill[g] = inert ? -1 : (int)(unsigned)v;
+          } else {
+            float out = v;
+            if (inert) {
+              const float raw = a.src[g];
+              if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
+            }
+            a.state[g] = (MODE == kModeRecon) ? -out : out;
+          }
+        }
       }
     }
     // ---- changed perimeter -> neighbour tiles for the next round
@@ -178,90 +270,71 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
     }
     if (bits) atomicOr(&s_bits, bits);
     __syncthreads();
-    if (tid == 0) {
-      queue_neighbours(s_bits, tx, ty, a.ntx, a.nty, a.mark, a.list, a.sync, round + 1);
-      atomicAdd(&a.sync->visits, 1ull);
-      atomicAdd(&a.sync->local_iters, (unsigned long long)it);
-      atomicMax(&a.sync->max_local, it);
-      if (any) atomicOr(&a.sync->error, kErrLocalCap);
-    }
+    if (tid == 0)
+      finish_visit(a.sync, a.mark, a.list, s_bits, tx, ty, a.ntx, a.nty, round, it, any != 0,
+                   c0, c1, c2);
     __syncthreads();
   };
 
-  for (unsigned round = 0;; round++) {
-    if (round == 0) {
-      for (int t = blockIdx.x; t < ntiles; t += nb) visit(t, 0u);
-    } else {
-      const unsigned n = ld_relaxed(&a.sync->count[round & 1u]);
-      const unsigned* L = a.list + (size_t)(round & 1u) * ntiles;
-      for (unsigned k = blockIdx.x; k < n; k += nb) visit((int)L[k], round);
-    }
-    const unsigned consumed = round & 1u;
-    bool ok = grid_barrier(a.sync, nb, [&] {
-      __hip_atomic_store(&a.sync->count[consumed], 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    });
-    if (!ok) return;
-    const unsigned next = ld_relaxed(&a.sync->count[(round + 1) & 1u]);
-    if (next == 0u || round + 1 >= a.max_rounds) {
-      if (blockIdx.x == 0 && tid == 0) {
-        a.sync->rounds = round + 1;
-        if (next != 0u) atomicOr(&a.sync->error, kErrRoundCap);
-      }
-      return;
-    }
+  run_rounds(a.sync, a.list, ntiles, a.max_rounds, visit);
+}
+
+template <class G, int MODE>
+static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats) {
+  a.ntx = (int)((a.nx + G::TX - 1) / G::TX);
+  a.nty = (int)((a.ny + G::TY - 1) / G::TY);
+  a.max_rounds = 1u << 20;
+  a.max_local = 1u << 16;
+  return launch_persistent(fill_persistent<G, MODE>, G::THREADS, a.ntx * a.nty, a, stream, stats,
+                           G::TX, G::TY);
+}
+
+template <int MODE>
+static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, RunStats* stats) {
+  if constexpr (MODE == kModeSpill) {
+    // 64-bit keys: at most two quadrants fit LDS
+    if (ts.qx == 1 && ts.qy == 1) return launch_fill<TileGeom<1, 1>, MODE>(a, stream, stats);
+    if (ts.qx == 1 && ts.qy == 2) return launch_fill<TileGeom<1, 2>, MODE>(a, stream, stats);
+    return launch_fill<TileGeom<2, 1>, MODE>(a, stream, stats);
+  } else {
+    if (ts.qx == 2 && ts.qy == 2) return launch_fill<TileGeom<2, 2>, MODE>(a, stream, stats);
+    if (ts.qx == 2 && ts.qy == 1) return launch_fill<TileGeom<2, 1>, MODE>(a, stream, stats);
+    if (ts.qx == 1 && ts.qy == 2) return launch_fill<TileGeom<1, 2>, MODE>(a, stream, stats);
+    return launch_fill<TileGeom<1, 1>, MODE>(a, stream, stats);
   }
 }
 
-template <class G>
-static hipError_t launch_fill(float* state, const float* src, const uint8_t* bc, int64_t nx,
-                              int64_t ny, int mode, hipStream_t stream, RunStats* stats) {
+hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
+                    int mode, TileShape ts, hipStream_t stream, RunStats* stats) {
+  if (nx <= 0 || ny <= 0) return hipSuccess;
   FillArgs a{};
   a.state = state;
   a.src = src;
   a.bc = bc;
   a.nx = nx;
   a.ny = ny;
-  a.ntx = (int)((nx + G::TX - 1) / G::TX);
-  a.nty = (int)((ny + G::TY - 1) / G::TY);
-  a.max_rounds = 1u << 20;
-  a.max_local = 1u << 16;
-  const int ntiles = a.ntx * a.nty;
-  const void* kern = mode == kModeFill ? (const void*)fill_persistent<G, kModeFill>
-                                       : (const void*)fill_persistent<G, kModeRecon>;
-  int grid = 0;
-  hipError_t e = persistent_grid(kern, G::THREADS, ntiles, &grid);
-  if (e != hipSuccess) return e;
-  Workspace ws;
-  e = ws.alloc(ntiles, stream);
-  if (e != hipSuccess) return e;
-  a.mark = ws.mark;
-  a.list = ws.list;
-  a.sync = ws.sync;
-  if (mode == kModeFill)
-    fill_persistent<G, kModeFill><<<grid, G::THREADS, 0, stream>>>(a);
-  else
-    fill_persistent<G, kModeRecon><<<grid, G::THREADS, 0, stream>>>(a);
-  e = hipGetLastError();
-  if (stats) {
-    stats->grid = grid;
-    stats->tile_x = G::TX;
-    stats->tile_y = G::TY;
-  }
-  hipError_t f = ws.finish(stats);
-  return e != hipSuccess ? e : f;
+  if (mode == kModeFill) return dispatch_fill<kModeFill>(a, ts, stream, stats);
+  return dispatch_fill<kModeRecon>(a, ts, stream, stats);
 }
 
-hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
-                    int mode, TileShape ts, hipStream_t stream, RunStats* stats) {
+hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc,
+                          int64_t nx, int64_t ny, TileShape ts, hipStream_t stream,
+                          RunStats* stats) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
-  if (ts.qx == 2 && ts.qy == 2)
-    return launch_fill<TileGeom<2, 2>>(state, src, bc, nx, ny, mode, stream, stats);
-  if (ts.qx == 2 && ts.qy == 1)
-    return launch_fill<TileGeom<2, 1>>(state, src, bc, nx, ny, mode, stream, stats);
-  if (ts.qx == 1 && ts.qy == 2)
-    return launch_fill<TileGeom<1, 2>>(state, src, bc, nx, ny, mode, stream, stats);
-  return launch_fill<TileGeom<1, 1>>(state, src, bc, nx, ny, mode, stream, stats);
+  FillArgs a{};
+  a.state = out;
+  a.spill = spill;
+  a.src = dem;
+  a.bc = bc;
+  a.nx = nx;
+  a.ny = ny;
+  // (level, index) keys live in one 64-bit array between rounds; the
+  // separate level and index outputs are written alongside
+  hipError_t e = hipMallocAsync((void**)&a.keys, (size_t)nx * ny * 8, stream);
+  if (e != hipSuccess) return e;
+  e = dispatch_fill<kModeSpill>(a, ts, stream, stats);
+  hipError_t f = hipFreeAsync(a.keys, stream);
+  return e != hipSuccess ? e : f;
 }
 
 }  // namespace tt

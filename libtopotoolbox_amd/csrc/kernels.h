@@ -14,6 +14,8 @@ struct RunStats {
   int64_t grid = 0;         // workgroups in the persistent grid
   int64_t tile_x = 0, tile_y = 0;
   uint32_t error = 0;       // Sync error bits
+  // wall-clock ticks (100 MHz) summed over workgroups, by phase
+  int64_t t_stage = 0, t_relax = 0, t_write = 0, t_wait = 0;
 };
 
 // Tile shape in 64x64 quadrants (one wavefront per quadrant).
@@ -34,6 +36,22 @@ hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t n
 // Fast-sweeping excess topography: u (output) from dem and slopes.
 hipError_t run_fsm(float* u, const float* dem, const float* slopes, float cellsize, int64_t nx,
                    int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
+
+// fillsinks plus the spill cell of every output value (int32, -1 for NaN
+// and unreached cells); needs nx * ny <= INT32_MAX.
+hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc, int64_t nx,
+                          int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
+
+// g_dem[q] = sum of g_out[p] over p with spill[p] == q (deterministic).
+// `nonfinite` (host, may be null) receives 1 if g_out held NaN / INF, which
+// contribute nothing; non-null synchronises the stream.
+hipError_t run_fill_backward(float* g_dem, const int* spill, const float* g_out, int64_t nx,
+                             int64_t ny, hipStream_t stream, uint32_t* nonfinite);
+
+// Adjoint of excesstopography_fsm2d at its converged output u.
+hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const float* dem,
+                            const float* slopes, float cellsize, const float* g_out, int64_t nx,
+                            int64_t ny, hipStream_t stream, RunStats* stats);
 
 // Thread-local error message set by the entry points.
 void set_error(const char* fmt, ...);

@@ -33,6 +33,11 @@ struct Sync {
   unsigned int pad;
   unsigned long long visits;       // tile visits
   unsigned long long local_iters;  // sum of local iterations over visits
+  // wall-clock ticks (100 MHz) summed over workgroups, by phase
+  unsigned long long t_stage;      // tile + halo staging into LDS
+  unsigned long long t_relax;      // local sweeps
+  unsigned long long t_write;      // write-back and neighbour queueing
+  unsigned long long t_wait;       // grid barrier waits
 };
 
 enum : unsigned int {
@@ -164,6 +169,57 @@ __device__ inline void queue_neighbours(unsigned bits, int tx, int ty, int ntx, 
     if (x < 0 || x >= ntx || y < 0 || y >= nty) continue;
     queue_tile(mark, list, s, ntiles, y * ntx + x, next);
   }
+}
+
+// The persistent round loop. `visit(t, round)` processes tile t (all
+// threads of the workgroup call it together) and queues the neighbours it
+// changed for round + 1. Round 0 visits every tile; round r > 0 the tiles
+// queued during round r - 1. Returns when a round queues nothing.
+template <class Visit>
+__device__ inline void run_rounds(Sync* s, const unsigned* list, int ntiles,
+                                  unsigned max_rounds, Visit visit) {
+  const unsigned nb = gridDim.x;
+  for (unsigned round = 0;; round++) {
+    if (round == 0) {
+      for (int t = blockIdx.x; t < ntiles; t += nb) visit(t, 0u);
+    } else {
+      const unsigned n = ld_relaxed(&s->count[round & 1u]);
+      const unsigned* L = list + (size_t)(round & 1u) * ntiles;
+      for (unsigned k = blockIdx.x; k < n; k += nb) visit((int)L[k], round);
+    }
+    const unsigned consumed = round & 1u;
+    const unsigned long long w0 = wall_clock64();
+    bool ok = grid_barrier(s, nb, [&] {
+      __hip_atomic_store(&s->count[consumed], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    });
+    if (!ok) return;
+    if (threadIdx.x == 0) atomicAdd(&s->t_wait, wall_clock64() - w0);
+    const unsigned next = ld_relaxed(&s->count[(round + 1) & 1u]);
+    if (next == 0u || round + 1 >= max_rounds) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->rounds = round + 1;
+        if (next != 0u) atomicOr(&s->error, kErrRoundCap);
+      }
+      return;
+    }
+  }
+}
+
+// Per-visit bookkeeping by thread 0: queue neighbours, counters, phase
+// timers (c0..c3: start, staged, relaxed, written).
+__device__ inline void finish_visit(Sync* s, unsigned* mark, unsigned* list, unsigned bits,
+                                    int tx, int ty, int ntx, int nty, unsigned round,
+                                    unsigned it, bool capped, unsigned long long c0,
+                                    unsigned long long c1, unsigned long long c2) {
+  queue_neighbours(bits, tx, ty, ntx, nty, mark, list, s, round + 1);
+  atomicAdd(&s->visits, 1ull);
+  atomicAdd(&s->local_iters, (unsigned long long)it);
+  atomicMax(&s->max_local, it);
+  if (capped) atomicOr(&s->error, kErrLocalCap);
+  const unsigned long long c3 = wall_clock64();
+  atomicAdd(&s->t_stage, c1 - c0);
+  atomicAdd(&s->t_relax, c2 - c1);
+  atomicAdd(&s->t_write, c3 - c2);
 }
 
 }  // namespace tt

@@ -30,4 +30,30 @@ struct Workspace {
 // Workgroups of the persistent grid: co-resident blocks, capped by tiles.
 hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* grid);
 
+// Allocates the workspace, launches `kern` as a persistent grid over
+// `ntiles` tiles (Args must carry mark / list / sync) and releases the
+// workspace stream-ordered; fills `stats` (synchronising) when non-null.
+template <class Args>
+hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a,
+                             hipStream_t stream, RunStats* stats, int tile_x, int tile_y) {
+  int grid = 0;
+  hipError_t e = persistent_grid((const void*)kern, threads, ntiles, &grid);
+  if (e != hipSuccess) return e;
+  Workspace ws;
+  e = ws.alloc(ntiles, stream);
+  if (e != hipSuccess) return e;
+  a.mark = ws.mark;
+  a.list = ws.list;
+  a.sync = ws.sync;
+  kern<<<grid, threads, 0, stream>>>(a);
+  e = hipGetLastError();
+  if (stats) {
+    stats->grid = grid;
+    stats->tile_x = tile_x;
+    stats->tile_y = tile_y;
+  }
+  hipError_t f = ws.finish(stats);
+  return e != hipSuccess ? e : f;
+}
+
 }  // namespace tt

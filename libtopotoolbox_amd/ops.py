@@ -118,6 +118,59 @@ def excesstopography_fsm2d(dem: torch.Tensor, threshold_slopes: torch.Tensor, ce
     return _ret(out, st)
 
 
+def fillsinks_spill(dem: torch.Tensor, bc: torch.Tensor, stats: bool = False):
+    """Filled DEM and, per cell, the int32 index (p = j * nx + i) of the cell
+    whose DEM value the output copies; -1 on NaN and unreached cells."""
+    _check_grid(dem, "dem")
+    _check_grid(bc, "bc", torch.uint8)
+    _same_shape(dem, ("bc", bc))
+    out = torch.empty_like(dem)
+    spill = torch.empty(dem.shape, dtype=torch.int32, device=dem.device)
+    ny, nx = dem.shape
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_fillsinks_spill_dev(out.data_ptr(), spill.data_ptr(), dem.data_ptr(),
+                                               bc.data_ptr(), nx, ny, _stream_handle(), st)
+    _lib.check(rc, "fillsinks_spill")
+    return _ret((out, spill), st)
+
+
+def fillsinks_backward(spill: torch.Tensor, g_out: torch.Tensor) -> torch.Tensor:
+    """g_dem[q] = sum of g_out[p] over p with spill[p] == q (deterministic)."""
+    _check_grid(spill, "spill", torch.int32)
+    _check_grid(g_out, "g_out")
+    _same_shape(spill, ("g_out", g_out))
+    g_dem = torch.empty_like(g_out)
+    ny, nx = g_out.shape
+    with torch.cuda.device(g_out.device):
+        rc = _lib.lib().tt_fillsinks_backward_dev(g_dem.data_ptr(), spill.data_ptr(),
+                                                  g_out.data_ptr(), nx, ny, _stream_handle())
+    _lib.check(rc, "fillsinks_backward")
+    return g_dem
+
+
+def excesstopography_fsm2d_backward(excess: torch.Tensor, dem: torch.Tensor,
+                                    threshold_slopes: torch.Tensor, cellsize: float,
+                                    g_out: torch.Tensor, stats: bool = False):
+    """(dL/ddem, dL/dslopes) for L = sum(g_out * excess) at the converged
+    output `excess` of excesstopography_fsm2d."""
+    for name, t in (("excess", excess), ("dem", dem), ("threshold_slopes", threshold_slopes),
+                    ("g_out", g_out)):
+        _check_grid(t, name)
+    _same_shape(excess, ("dem", dem), ("threshold_slopes", threshold_slopes), ("g_out", g_out))
+    g_dem = torch.empty_like(dem)
+    g_s = torch.empty_like(dem)
+    ny, nx = dem.shape
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_excesstopography_fsm2d_backward_dev(
+            g_dem.data_ptr(), g_s.data_ptr(), excess.data_ptr(), dem.data_ptr(),
+            threshold_slopes.data_ptr(), float(cellsize), g_out.data_ptr(), nx, ny,
+            _stream_handle(), st)
+    _lib.check(rc, "excesstopography_fsm2d_backward")
+    return _ret((g_dem, g_s), st)
+
+
 # ------------------------------------------------------------------ host drop-ins
 def _host(a, dtype) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=dtype)
