@@ -75,7 +75,7 @@ def gpu_run(workload: str, arrays: dict, steps: int, warmup: int):
         def step(stats=False):
             return tt.excesstopography_fsm2d(dem, slopes, 30.0, out=out, stats=stats)
 
-    _, st = step(stats=True)  # also a warmup; reports rounds / visits
+    _, st = step(stats=True)  # also a warmup; reports visits and phase times
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -195,7 +195,7 @@ def main():
                                                          r["stats"]["tile_y"]]},
         "roofline": roofline(args.workload, cells, launch_ms,
                              measured_traffic(args.workload)),
-        "kernel_stats": {k: r["stats"][k] for k in ("rounds", "visits", "local_iters",
+        "kernel_stats": {k: r["stats"][k] for k in ("requeues", "visits", "local_iters",
                                                     "max_local", "grid")},
     }
     del r
@@ -210,7 +210,7 @@ def main():
               "gcells_per_s": round(cells / (ms2 * 1e-3) / 1e9, 4),
               "roofline": roofline(other, cells, float(np.mean(r2["launch_ms"])),
                                    measured_traffic(other)),
-              "kernel_stats": {k: r2["stats"][k] for k in ("rounds", "visits", "local_iters",
+              "kernel_stats": {k: r2["stats"][k] for k in ("requeues", "visits", "local_iters",
                                                            "max_local", "grid")}}
         if not args.no_cpu:
             ex["cpu_baseline"] = cpu_baseline(other, a2, min(2048, n))

@@ -68,8 +68,8 @@ void excesstopography_fsm2d(float *excess, float *dem, float *threshold_slopes,
    --------------------------------------------------------------------- */
 
 typedef struct tt_stats {
-  int64_t rounds;      /* persistent-grid rounds until no tile changed */
-  int64_t visits;      /* tile visits summed over rounds */
+  int64_t requeues;    /* tiles queued again after the initial queue of all tiles */
+  int64_t visits;      /* tile visits (all tiles once, plus the requeues) */
   int64_t local_iters; /* local sweep iterations summed over visits */
   int64_t max_local;   /* largest local iteration count of one visit */
   int64_t grid;        /* workgroups of the persistent launch */

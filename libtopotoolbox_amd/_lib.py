@@ -24,7 +24,7 @@ class Stats(ctypes.Structure):
     """Mirror of tt_stats."""
 
     _fields_ = [
-        ("rounds", ctypes.c_int64),
+        ("requeues", ctypes.c_int64),
         ("visits", ctypes.c_int64),
         ("local_iters", ctypes.c_int64),
         ("max_local", ctypes.c_int64),

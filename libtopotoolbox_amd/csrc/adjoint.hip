@@ -197,10 +197,7 @@ struct AdjArgs {
   const float* sy;
   long long nx, ny;
   int ntx, nty;
-  unsigned* mark;
-  unsigned* list;
-  Sync* sync;
-  unsigned max_rounds;
+  TileQueue q;
   unsigned max_local;
 };
 
@@ -276,58 +273,25 @@ __global__ __launch_bounds__(G::THREADS) void fsm_adjoint_persistent(AdjArgs a) 
   const int wave = tid >> 6;
   const int R0 = (wave / G::QX) * 64;
   const int C0 = (wave % G::QX) * 64;
-  const int ntiles = a.ntx * a.nty;
+  const unsigned* sl_bits = reinterpret_cast<const unsigned*>(sl);
 
-  auto visit = [&](int t, unsigned round) {
+  auto visit = [&](int t, bool first, unsigned nvis) -> unsigned {
     const unsigned long long c0 = wall_clock64();
     const int tx = t % a.ntx, ty = t / a.ntx;
     const long long x0 = (long long)tx * G::TX, y0 = (long long)ty * G::TY;
-    constexpr int E = G::ROWS * (G::TX + 2);
-    constexpr int U = 4;
-    for (int base = 0; base < E; base += G::THREADS * U) {
-      float lv[U], gv[U], xv[U], yv[U];
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        const int idx = base + v * G::THREADS + tid;
-        const int r = idx / (G::TX + 2) - 1;
-        const int c = idx % (G::TX + 2) - 1;
-        const long long gx = x0 + c, gy = y0 + r;
-        float l = 0.f, g = 0.f, x = 0.f, y = 0.f;
-        if (idx < E && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
-          const size_t q = (size_t)gy * a.nx + gx;
-          g = a.g[q];
-          l = (round == 0) ? g : a.lam[q];
-          x = a.sx[q];
-          y = a.sy[q];
-        }
-        lv[v] = l;
-        gv[v] = g;
-        xv[v] = x;
-        yv[v] = y;
-      }
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        const int idx = base + v * G::THREADS + tid;
-        if (idx < E) {
-          const int r = idx / (G::TX + 2) - 1;
-          const int c = idx % (G::TX + 2) - 1;
-          sl[G::at(r, c)] = lv[v];
-          sg[G::at(r, c)] = gv[v];
-          ssx[G::at(r, c)] = xv[v];
-          ssy[G::at(r, c)] = yv[v];
-        }
-      }
-    }
-    if (tid == 0) s_bits = 0u;
+    // lambda starts at g; outside the grid everything is 0 (no pulls)
+    stage_tile<G>(StageF32{first ? a.g : a.lam, sl, 0.f}, StageF32{a.g, sg, 0.f}, true, x0, y0,
+                  a.nx, a.ny);
+    stage_tile<G>(StageF32{a.sx, ssx, 0.f}, StageF32{a.sy, ssy, 0.f}, true, x0, y0, a.nx, a.ny);
     __syncthreads();
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      sper[k] = __float_as_uint(sl[G::at(0, k)]);
-      sper[G::TX + k] = __float_as_uint(sl[G::at(G::TY - 1, k)]);
-    }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      sper[2 * G::TX + k] = __float_as_uint(sl[G::at(k, 0)]);
-      sper[2 * G::TX + G::TY + k] = __float_as_uint(sl[G::at(k, G::TX - 1)]);
-    }
+    for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx < 0 || gx >= a.nx || gy < 0 || gy >= a.ny) return;
+      const size_t q = (size_t)gy * a.nx + gx;
+      sl[G::at(r, c)] = from_state ? a.lam[q] : a.g[q];
+    });
+    __syncthreads();
+    save_perimeter<G>(sper, sl_bits);
     const unsigned long long c1 = wall_clock64();
     unsigned it = 0;
     int any, any_change = 0;
@@ -341,32 +305,19 @@ __global__ __launch_bounds__(G::THREADS) void fsm_adjoint_persistent(AdjArgs a) 
       it++;
     } while (any && it < a.max_local);
     const unsigned long long c2 = wall_clock64();
-    if (round == 0 || any_change) {
+    if (first || any_change) {
       for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
         const int r = idx / G::TX, c = idx % G::TX;
         const long long gx = x0 + c, gy = y0 + r;
         if (gx < a.nx && gy < a.ny) a.lam[(size_t)gy * a.nx + gx] = sl[G::at(r, c)];
       }
     }
-    unsigned bits = 0;
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      if (sper[k] != __float_as_uint(sl[G::at(0, k)])) bits |= kEdgeN;
-      if (sper[G::TX + k] != __float_as_uint(sl[G::at(G::TY - 1, k)])) bits |= kEdgeS;
-    }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      if (sper[2 * G::TX + k] != __float_as_uint(sl[G::at(k, 0)])) bits |= kEdgeW;
-      if (sper[2 * G::TX + G::TY + k] != __float_as_uint(sl[G::at(k, G::TX - 1)]))
-        bits |= kEdgeE;
-    }
-    if (bits) atomicOr(&s_bits, bits);
-    __syncthreads();
-    if (tid == 0)
-      finish_visit(a.sync, a.mark, a.list, s_bits, tx, ty, a.ntx, a.nty, round, it, any != 0,
-                   c0, c1, c2);
-    __syncthreads();
+    const unsigned bits = block_or(changed_perimeter<G, false>(sper, sl_bits), &s_bits);
+    if (tid == 0) visit_stats(a.q.sync, it, any != 0, c0, c1, c2);
+    return bits;
   };
 
-  run_rounds(a.sync, a.list, ntiles, a.max_rounds, visit);
+  run_queue(a.q, a.ntx, a.nty, visit);
 }
 
 hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const float* dem,
@@ -395,7 +346,6 @@ hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const
     a.ny = ny;
     a.ntx = (int)((nx + G::TX - 1) / G::TX);
     a.nty = (int)((ny + G::TY - 1) / G::TY);
-    a.max_rounds = 1u << 20;
     a.max_local = 1u << 16;
     e = launch_persistent(fsm_adjoint_persistent<G>, G::THREADS, a.ntx * a.nty, a, stream, stats,
                           G::TX, G::TY);

@@ -15,7 +15,7 @@ namespace {
 thread_local tt::TileShape g_shape;
 
 void to_public(const tt::RunStats& s, tt_stats* o) {
-  o->rounds = s.rounds;
+  o->requeues = s.requeues;
   o->visits = s.visits;
   o->local_iters = s.local_iters;
   o->max_local = s.max_local;

@@ -14,14 +14,14 @@
 // as fmaxf skips NaN in the reference), written back as the input NaN.
 //
 // fillsinks:   d = dem, f0 = (bc == 1) ? dem : +INF; the output array
-//              holds f between rounds.
+//              holds f between visits.
 // reconstruct: d = -mask, f0 = max(d, -marker) (a NaN marker acts as -INF);
-//              the marker array holds -f between rounds and at the end.
+//              the marker array holds -f between visits and at the end.
 // spill:       fillsinks over lexicographic (level, cell index) keys packed
 //              in 64 bits; the index half of the fixed point is the cell
 //              whose DEM value the output copies (the spill cell), which
 //              the adjoint scatters to. Unique fixed point, so deterministic.
-//              Halo reads race with neighbour write-backs within a round,
+//              Halo reads race with neighbour write-backs,
 //              which is harmless only because every read value is a valid
 //              upper bound: hence one 64-bit word per key, never two arrays.
 #include <type_traits>
@@ -42,10 +42,7 @@ struct FillArgs {
   const uint8_t* bc;
   long long nx, ny;
   int ntx, nty;
-  unsigned* mark;
-  unsigned* list;
-  Sync* sync;
-  unsigned max_rounds;
+  TileQueue q;
   unsigned max_local;
 };
 
@@ -148,70 +145,92 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
   const int wave = tid >> 6;
   const int R0 = (wave / G::QX) * 64;
   const int C0 = (wave % G::QX) * 64;
-  const int ntiles = a.ntx * a.nty;
 
-  auto visit = [&](int t, unsigned round) {
+  auto visit = [&](int t, bool first, unsigned nvis) -> unsigned {
     const unsigned long long c0 = wall_clock64();
     const int tx = t % a.ntx, ty = t / a.ntx;
     const long long x0 = (long long)tx * G::TX, y0 = (long long)ty * G::TY;
-    // ---- stage tile + halo: U independent global loads in flight per
-    // thread before the LDS stores
-    constexpr int E = G::ROWS * (G::TX + 2);
-    constexpr int U = 8;
-    for (int base = 0; base < E; base += G::THREADS * U) {
-      T fv[U], dv[U];
+    // ---- stage tile + halo
+    if constexpr (MODE == kModeSpill) {
+      constexpr int E = G::ROWS * (G::TX + 2);
+      constexpr int U = 8;
+      for (int base = 0; base < E; base += G::THREADS * U) {
+        T fv[U], dv[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        const int r = idx / (G::TX + 2) - 1;
-        const int c = idx % (G::TX + 2) - 1;
-        const long long gx = x0 + c, gy = y0 + r;
-        T f = L::kUnreached, d = L::kInert;
-        if (idx < E && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
-          const size_t g = (size_t)gy * a.nx + gx;
-          const float raw = a.src[g];
-          if constexpr (MODE == kModeFill) {
-            if (raw == raw) {
-              d = raw;
-              f = (round == 0) ? ((a.bc[g] == 1) ? raw : kInf) : a.state[g];
-            }
-          } else if constexpr (MODE == kModeRecon) {
-            if (raw == raw) {
-              d = -raw;
-              const float m = a.state[g];
-              f = fmaxf(d, (m == m) ? -m : kInf);
-            }
-          } else {
-            if (raw == raw) {
-              d = KeyLevel::make(raw, (unsigned)g);
-              f = (round == 0) ? ((a.bc[g] == 1) ? d : KeyLevel::kUnreached) : a.keys[g];
-            }
-          }
-        }
-        fv[u] = f;
-        dv[u] = d;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        if (idx < E) {
+        for (int u = 0; u < U; u++) {
+          const int idx = base + u * G::THREADS + tid;
           const int r = idx / (G::TX + 2) - 1;
           const int c = idx % (G::TX + 2) - 1;
-          sf[G::at(r, c)] = fv[u];
-          sd[G::at(r, c)] = dv[u];
+          const long long gx = x0 + c, gy = y0 + r;
+          T f = L::kUnreached, d = L::kInert;
+          if (idx < E && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
+            const size_t g = (size_t)gy * a.nx + gx;
+            const float raw = a.src[g];
+            if (raw == raw) {
+              d = KeyLevel::make(raw, (unsigned)g);
+              f = first ? ((a.bc[g] == 1) ? d : KeyLevel::kUnreached) : a.keys[g];
+            }
+          }
+          fv[u] = f;
+          dv[u] = d;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int idx = base + u * G::THREADS + tid;
+          if (idx < E) {
+            const int r = idx / (G::TX + 2) - 1;
+            const int c = idx % (G::TX + 2) - 1;
+            sf[G::at(r, c)] = fv[u];
+            sd[G::at(r, c)] = dv[u];
+          }
         }
       }
+    } else {
+      // raw inputs first (bound source in sd; bc flag or stored state in sf),
+      // then one LDS pass turns them into (level, bound)
+      const StageF32 src{a.src, sd, __builtin_nanf("")};
+      if (MODE == kModeFill && first)
+        stage_tile<G>(src, StageU8{a.bc, sf, 0.f}, true, x0, y0, a.nx, a.ny);
+      else
+        stage_tile<G>(src, StageF32{a.state, sf, kInf}, true, x0, y0, a.nx, a.ny);
+      __syncthreads();
+      for (int k = tid; k < G::CELLS; k += G::THREADS) {
+        float d = sd[k], f = sf[k];
+        if (MODE == kModeFill) {
+          if (first) f = (f == 1.f) ? d : kInf;
+        } else {
+          d = -d;
+          f = fmaxf(d, (f == f) ? -f : kInf);
+        }
+        if (d != d) {  // NaN input: inert cell
+          d = kInf;
+          f = kInf;
+        }
+        sd[k] = d;
+        sf[k] = f;
+      }
     }
-    if (tid == 0) s_bits = 0u;
     __syncthreads();
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      sper[k] = sf[G::at(0, k)];
-      sper[G::TX + k] = sf[G::at(G::TY - 1, k)];
+    // halo cells whose owner tile's stored state availability differs from
+    // what was staged (recon state is always initialised: the marker)
+    if (MODE != kModeRecon) {
+      for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
+        const long long gx = x0 + c, gy = y0 + r;
+        if (gx < 0 || gx >= a.nx || gy < 0 || gy >= a.ny) return;
+        const size_t g = (size_t)gy * a.nx + gx;
+        const float raw = a.src[g];
+        if (raw != raw) return;  // inert either way
+        if constexpr (MODE == kModeSpill) {
+          sf[G::at(r, c)] = from_state ? a.keys[g]
+                                       : ((a.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
+                                                         : KeyLevel::kUnreached);
+        } else {
+          sf[G::at(r, c)] = from_state ? a.state[g] : ((a.bc[g] == 1) ? raw : kInf);
+        }
+      });
+      __syncthreads();
     }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      sper[2 * G::TX + k] = sf[G::at(k, 0)];
-      sper[2 * G::TX + G::TY + k] = sf[G::at(k, G::TX - 1)];
-    }
+    save_perimeter<G>(sper, sf);
     const unsigned long long c1 = wall_clock64();
     // ---- relax to the local fixed point
     unsigned it = 0;
@@ -226,10 +245,10 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
       it++;
     } while (any && it < a.max_local);
     const unsigned long long c2 = wall_clock64();
-    // ---- write back: round 0 initialises the whole state; later rounds
-    // skip tiles whose relaxation changed nothing. Inert cells (NaN input)
-    // take the input bits.
-    if (round == 0 || any_change) {
+    // ---- write back: the first visit initialises the tile's state; later
+    // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
+    // input) take the input bits.
+    if (first || any_change) {
       for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
         const int r = idx / G::TX, c = idx % G::TX;
         const long long gx = x0 + c, gy = y0 + r;
@@ -254,36 +273,19 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
         }
       }
     }
-    // ---- changed perimeter -> neighbour tiles for the next round
-    unsigned bits = 0;
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      if (sper[k] != sf[G::at(0, k)])
-        bits |= kEdgeN | (k == 0 ? kEdgeNW : 0u) | (k == G::TX - 1 ? kEdgeNE : 0u);
-      if (sper[G::TX + k] != sf[G::at(G::TY - 1, k)])
-        bits |= kEdgeS | (k == 0 ? kEdgeSW : 0u) | (k == G::TX - 1 ? kEdgeSE : 0u);
-    }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      if (sper[2 * G::TX + k] != sf[G::at(k, 0)])
-        bits |= kEdgeW | (k == 0 ? kEdgeNW : 0u) | (k == G::TY - 1 ? kEdgeSW : 0u);
-      if (sper[2 * G::TX + G::TY + k] != sf[G::at(k, G::TX - 1)])
-        bits |= kEdgeE | (k == 0 ? kEdgeNE : 0u) | (k == G::TY - 1 ? kEdgeSE : 0u);
-    }
-    if (bits) atomicOr(&s_bits, bits);
-    __syncthreads();
-    if (tid == 0)
-      finish_visit(a.sync, a.mark, a.list, s_bits, tx, ty, a.ntx, a.nty, round, it, any != 0,
-                   c0, c1, c2);
-    __syncthreads();
+    // ---- changed perimeter -> neighbour tiles to queue
+    const unsigned bits = block_or(changed_perimeter<G, true>(sper, sf), &s_bits);
+    if (tid == 0) visit_stats(a.q.sync, it, any != 0, c0, c1, c2);
+    return bits;
   };
 
-  run_rounds(a.sync, a.list, ntiles, a.max_rounds, visit);
+  run_queue(a.q, a.ntx, a.nty, visit);
 }
 
 template <class G, int MODE>
 static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats) {
   a.ntx = (int)((a.nx + G::TX - 1) / G::TX);
   a.nty = (int)((a.ny + G::TY - 1) / G::TY);
-  a.max_rounds = 1u << 20;
   a.max_local = 1u << 16;
   return launch_persistent(fill_persistent<G, MODE>, G::THREADS, a.ntx * a.nty, a, stream, stats,
                            G::TX, G::TY);
@@ -328,7 +330,7 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   a.bc = bc;
   a.nx = nx;
   a.ny = ny;
-  // (level, index) keys live in one 64-bit array between rounds; the
+  // (level, index) keys live in one 64-bit array between visits; the
   // separate level and index outputs are written alongside
   hipError_t e = hipMallocAsync((void**)&a.keys, (size_t)nx * ny * 8, stream);
   if (e != hipSuccess) return e;

@@ -31,10 +31,7 @@ struct FsmArgs {
   float cellsize;
   long long nx, ny;
   int ntx, nty;
-  unsigned* mark;
-  unsigned* list;
-  Sync* sync;
-  unsigned max_rounds;
+  TileQueue q;
   unsigned max_local;
 };
 
@@ -113,53 +110,27 @@ __global__ __launch_bounds__(G::THREADS) void fsm_persistent(FsmArgs a) {
   const int wave = tid >> 6;
   const int R0 = (wave / G::QX) * 64;
   const int C0 = (wave % G::QX) * 64;
-  const int ntiles = a.ntx * a.nty;
+  const unsigned* su_bits = reinterpret_cast<const unsigned*>(su);
 
-  auto visit = [&](int t, unsigned round) {
+  auto visit = [&](int t, bool first, unsigned nvis) -> unsigned {
     const unsigned long long c0 = wall_clock64();
     const int tx = t % a.ntx, ty = t / a.ntx;
     const long long x0 = (long long)tx * G::TX, y0 = (long long)ty * G::TY;
-    constexpr int E = G::ROWS * (G::TX + 2);
-    constexpr int U = 8;
-    for (int base = 0; base < E; base += G::THREADS * U) {
-      float uv[U], fv[U];
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        const int idx = base + v * G::THREADS + tid;
-        const int r = idx / (G::TX + 2) - 1;
-        const int c = idx % (G::TX + 2) - 1;
-        const long long gx = x0 + c, gy = y0 + r;
-        float u = kInf, fi = __builtin_nanf("");
-        if (idx < E && gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny) {
-          const size_t g = (size_t)gy * a.nx + gx;
-          u = (round == 0) ? a.dem[g] : a.u[g];
-          fi = a.cellsize * a.slopes[g];
-        }
-        uv[v] = u;
-        fv[v] = fi;
-      }
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        const int idx = base + v * G::THREADS + tid;
-        if (idx < E) {
-          const int r = idx / (G::TX + 2) - 1;
-          const int c = idx % (G::TX + 2) - 1;
-          su[G::at(r, c)] = uv[v];
-          sfi[G::at(r, c)] = fv[v];
-        }
-      }
-    }
-    if (tid == 0) s_bits = 0u;
+    // u from the DEM on the first visit, from the stored state afterwards;
+    // fi = cellsize * slope (NaN outside the grid: never accepts a proposal)
+    stage_tile<G>(StageF32{first ? a.dem : a.u, su, kInf},
+                  StageF32{a.slopes, sfi, __builtin_nanf("")}, true, x0, y0, a.nx, a.ny);
     __syncthreads();
+    for (int k = tid; k < G::CELLS; k += G::THREADS) sfi[k] = a.cellsize * sfi[k];
+    for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx < 0 || gx >= a.nx || gy < 0 || gy >= a.ny) return;
+      const size_t g = (size_t)gy * a.nx + gx;
+      su[G::at(r, c)] = from_state ? a.u[g] : a.dem[g];
+    });
+    __syncthreads();
+    save_perimeter<G>(sper, su_bits);
     const unsigned long long c1 = wall_clock64();
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      sper[k] = __float_as_uint(su[G::at(0, k)]);
-      sper[G::TX + k] = __float_as_uint(su[G::at(G::TY - 1, k)]);
-    }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      sper[2 * G::TX + k] = __float_as_uint(su[G::at(k, 0)]);
-      sper[2 * G::TX + G::TY + k] = __float_as_uint(su[G::at(k, G::TX - 1)]);
-    }
     unsigned it = 0;
     int any, any_change = 0;
     do {
@@ -172,40 +143,28 @@ __global__ __launch_bounds__(G::THREADS) void fsm_persistent(FsmArgs a) {
       it++;
     } while (any && it < a.max_local);
     const unsigned long long c2 = wall_clock64();
-    // round 0 initialises the output; later rounds write changed tiles only
-    if (round == 0 || any_change) {
+    // the first visit initialises the output; later visits write changed
+    // tiles only
+    if (first || any_change) {
       for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
         const int r = idx / G::TX, c = idx % G::TX;
         const long long gx = x0 + c, gy = y0 + r;
         if (gx < a.nx && gy < a.ny) a.u[(size_t)gy * a.nx + gx] = su[G::at(r, c)];
       }
     }
-    unsigned bits = 0;
-    for (int k = tid; k < G::TX; k += G::THREADS) {
-      if (sper[k] != __float_as_uint(su[G::at(0, k)])) bits |= kEdgeN;
-      if (sper[G::TX + k] != __float_as_uint(su[G::at(G::TY - 1, k)])) bits |= kEdgeS;
-    }
-    for (int k = tid; k < G::TY; k += G::THREADS) {
-      if (sper[2 * G::TX + k] != __float_as_uint(su[G::at(k, 0)])) bits |= kEdgeW;
-      if (sper[2 * G::TX + G::TY + k] != __float_as_uint(su[G::at(k, G::TX - 1)]))
-        bits |= kEdgeE;
-    }
-    if (bits) atomicOr(&s_bits, bits);
-    __syncthreads();
-    if (tid == 0)
-      finish_visit(a.sync, a.mark, a.list, s_bits, tx, ty, a.ntx, a.nty, round, it, any != 0,
-                   c0, c1, c2);
-    __syncthreads();
+    // NaN cells compare as words, so they never count as changed
+    const unsigned bits = block_or(changed_perimeter<G, false>(sper, su_bits), &s_bits);
+    if (tid == 0) visit_stats(a.q.sync, it, any != 0, c0, c1, c2);
+    return bits;
   };
 
-  run_rounds(a.sync, a.list, ntiles, a.max_rounds, visit);
+  run_queue(a.q, a.ntx, a.nty, visit);
 }
 
 template <class G>
 static hipError_t launch_fsm(FsmArgs a, hipStream_t stream, RunStats* stats) {
   a.ntx = (int)((a.nx + G::TX - 1) / G::TX);
   a.nty = (int)((a.ny + G::TY - 1) / G::TY);
-  a.max_rounds = 1u << 20;
   a.max_local = 1u << 16;
   return launch_persistent(fsm_persistent<G>, G::THREADS, a.ntx * a.nty, a, stream, stats, G::TX,
                            G::TY);

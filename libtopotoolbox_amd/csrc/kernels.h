@@ -7,7 +7,7 @@
 namespace tt {
 
 struct RunStats {
-  int64_t rounds = 0;       // persistent-grid rounds
+  int64_t requeues = 0;     // tiles queued again after the initial queue
   int64_t visits = 0;       // tile visits
   int64_t local_iters = 0;  // local sweep iterations summed over visits
   int64_t max_local = 0;    // largest local iteration count of one visit

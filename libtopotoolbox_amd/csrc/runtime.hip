@@ -13,19 +13,39 @@ __global__ void zero_words(unsigned* p, size_t n) {
   for (; i < n; i += stride) p[i] = 0u;
 }
 
+__global__ void init_queue(Sync* s, unsigned long long* ring, unsigned* state,
+                           unsigned* visited, int ntiles) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    unsigned* w = reinterpret_cast<unsigned*>(s);
+    for (unsigned k = 0; k < sizeof(Sync) / 4; k++) w[k] = 0u;
+    s->tail = (unsigned)ntiles;
+    s->pending = (unsigned)ntiles;
+  }
+  for (int k = i; k < ntiles; k += gridDim.x * blockDim.x) {
+    ring[k] = ((unsigned long long)k << 32) | ((unsigned)k + 1u);
+    state[k] = kQueued;
+    visited[k] = 0u;
+  }
+}
+
 hipError_t Workspace::alloc(int ntiles, hipStream_t s) {
   stream = s;
-  // Sync block (rounded to 64 words) + mark[ntiles] + list[2 * ntiles]
+  // Sync block (rounded to 64 words) + ring (2 words per tile) + state and
+  // visited (1 word per tile)
   const size_t head = 64;
-  words = head + (size_t)ntiles * 3;
+  const size_t words = head + (size_t)ntiles * 4;
   hipError_t e = hipMallocAsync(&base, words * sizeof(unsigned), s);
   if (e != hipSuccess) return e;
-  sync = reinterpret_cast<Sync*>(base);
-  mark = reinterpret_cast<unsigned*>(base) + head;
-  list = mark + ntiles;
-  int blocks = (int)((words + 255) / 256);
+  q.sync = reinterpret_cast<Sync*>(base);
+  q.ring = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned*>(base) + head);
+  q.state = reinterpret_cast<unsigned*>(q.ring + ntiles);
+  q.visited = q.state + ntiles;
+  q.ntiles = ntiles;
+  int blocks = (ntiles + 255) / 256;
   if (blocks > 1024) blocks = 1024;
-  zero_words<<<blocks, 256, 0, s>>>(reinterpret_cast<unsigned*>(base), words);
+  if (blocks < 1) blocks = 1;
+  init_queue<<<blocks, 256, 0, s>>>(q.sync, q.ring, q.state, q.visited, ntiles);
   return hipGetLastError();
 }
 
@@ -33,10 +53,10 @@ hipError_t Workspace::finish(RunStats* st) {
   hipError_t e = hipSuccess;
   if (st) {
     Sync h;
-    e = hipMemcpyAsync(&h, sync, sizeof(Sync), hipMemcpyDeviceToHost, stream);
+    e = hipMemcpyAsync(&h, q.sync, sizeof(Sync), hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e == hipSuccess) {
-      st->rounds = h.rounds;
+      st->requeues = (int64_t)h.tail - (int64_t)q.ntiles;
       st->visits = (int64_t)h.visits;
       st->local_iters = (int64_t)h.local_iters;
       st->max_local = h.max_local;

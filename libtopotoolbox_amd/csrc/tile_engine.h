@@ -1,5 +1,5 @@
-// Persistent tiled relaxation engine shared by the sink-filling and the
-// excess-topography kernels (gfx950, wave64).
+// Persistent tiled relaxation engine shared by the sink-filling, the
+// excess-topography and the adjoint kernels (gfx950, wave64).
 //
 // A grid of nx x ny cells (x = dims[0] fastest) is cut into tiles of
 // TX x TY cells. One workgroup owns one tile at a time: it stages the tile
@@ -7,12 +7,25 @@
 // with directional sweeps (one 64x64 quadrant per wavefront), writes the
 // tile back and queues the neighbour tiles whose halo it changed.
 //
-// The grid is persistent: every workgroup loops over rounds. Round 0 visits
-// every tile; round r > 0 visits the tiles queued during round r-1. Rounds
-// are separated by a software grid barrier. The kernel ends when a round
-// queues nothing. All workgroups must be co-resident; the host sizes the
-// grid from the occupancy query, and every wait is bounded by a wall-clock
-// limit that turns a stuck barrier into an error flag instead of a hang.
+// Scheduling is a persistent grid over a lock-free FIFO of tile ids in
+// device memory. Each tile has a state word (QUEUED | RUNNING bits): every
+// tile starts queued; a workgroup pops a tile and marks it RUNNING (it is
+// then the tile's only owner), relaxes it, and pushes each neighbour whose
+// halo it changed. A push onto an idle tile enqueues it; a push onto a
+// running tile only sets QUEUED, and the owner, seeing it when it finishes,
+// relaxes the tile again at once. A counter of queued-or-running tiles
+// reaches zero exactly when the grid has converged, and every workgroup
+// then exits. There are no rounds and no grid barriers.
+//
+// All workgroups must be co-resident (the host sizes the grid from the
+// occupancy query). Every wait is bounded by a wall-clock limit that turns
+// a stuck wait into an error flag instead of a hang.
+//
+// Races: a tile's halo is read while neighbours may be writing their
+// tiles. The relaxations are monotone and every value a read can return is
+// a valid bound, and a neighbour that changes cells we read queues us
+// again, so the fixed point is reached regardless (state words must be read
+// and written whole: one aligned 32- or 64-bit word per cell).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,33 +35,31 @@ namespace tt {
 
 constexpr float kInf = __builtin_inff();
 
-// Control block in device memory, zeroed before each launch.
+// Control block in device memory, initialised before each launch.
 struct Sync {
-  unsigned int bar_count;   // arrivals at the current barrier
-  unsigned int bar_gen;     // barrier generation
+  unsigned int head;        // pop tickets handed out
+  unsigned int tail;        // push positions handed out
+  unsigned int pending;     // tiles queued or being relaxed
   unsigned int error;       // kErr* bits
-  unsigned int rounds;      // rounds executed (written by workgroup 0)
-  unsigned int count[2];    // queued tiles for the next round, by parity
-  unsigned int max_local;   // largest local iteration count of any visit
-  unsigned int pad;
+  unsigned int max_local;   // largest local iteration count of a visit
+  unsigned int pad[3];
   unsigned long long visits;       // tile visits
   unsigned long long local_iters;  // sum of local iterations over visits
   // wall-clock ticks (100 MHz) summed over workgroups, by phase
   unsigned long long t_stage;      // tile + halo staging into LDS
   unsigned long long t_relax;      // local sweeps
   unsigned long long t_write;      // write-back and neighbour queueing
-  unsigned long long t_wait;       // grid barrier waits
+  unsigned long long t_wait;       // waiting for a tile to pop
 };
 
 enum : unsigned int {
-  kErrBarrierTimeout = 1u,
-  kErrRoundCap = 2u,
+  kErrTimeout = 1u,
   kErrLocalCap = 4u,
 };
 
-// Upper bound on a single wait at a grid barrier (wall clock ticks of the
-// 100 MHz constant clock): 10 s.
-constexpr unsigned long long kBarrierTimeoutTicks = 1000000000ull;
+// Upper bound on one wait (wall-clock ticks of the 100 MHz constant
+// clock): 10 s.
+constexpr unsigned long long kWaitTimeoutTicks = 1000000000ull;
 
 __device__ inline unsigned ld_acquire(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -57,54 +68,180 @@ __device__ inline unsigned ld_relaxed(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Software grid barrier. Returns false (for every thread of the workgroup)
-// when the wait timed out or another workgroup reported an error. The last
-// workgroup to arrive runs `on_last` before releasing the others.
-template <class F>
-__device__ inline bool grid_barrier(Sync* s, unsigned nblocks, F on_last) {
-  __shared__ int ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ok = 1;
-    unsigned gen = ld_acquire(&s->bar_gen);
-    unsigned arrived = __hip_atomic_fetch_add(&s->bar_count, 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblocks - 1) {
-      on_last();
-      __hip_atomic_store(&s->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&s->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      unsigned long long t0 = wall_clock64();
-      while (ld_acquire(&s->bar_gen) == gen) {
-        __builtin_amdgcn_s_sleep(1);
-        if (ld_relaxed(&s->error) != 0u) {
-          ok = 0;
+// Work-queue state of one launch (device memory).
+struct TileQueue {
+  Sync* sync;
+  unsigned long long* ring;  // ntiles slots: position << 32 | (tile id + 1); 0 = empty
+  unsigned* state;    // per tile: kQueued | kRunning bits
+  unsigned* visited;  // per tile: 1 once its first visit has been written
+  int ntiles;
+};
+
+enum : unsigned { kQueued = 1u, kRunning = 2u };
+
+// Spinning uses relaxed loads only: an acquire load at agent scope
+// invalidates the XCD's L2, which hundreds of spinning workgroups would do
+// continuously. Ordering comes from one fence per visit in run_queue.
+
+// Push tile t (called by one thread, after a release fence that published
+// the caller's writes). The fetch_or always writes, so the tile's owner or
+// next popper, whose own read-modify-write of the state word follows it,
+// synchronises with it through the fences.
+__device__ inline void push_tile(const TileQueue& q, int t) {
+  const unsigned old =
+      __hip_atomic_fetch_or(&q.state[t], kQueued, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (old != 0u) return;  // already queued, or running (its owner will rerun it)
+  __hip_atomic_fetch_add(&q.sync->pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned pos =
+      __hip_atomic_fetch_add(&q.sync->tail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long* slot = &q.ring[pos % (unsigned)q.ntiles];
+  // the slot's previous entry (pos - ntiles) has a ticket holder, since at
+  // most ntiles tiles are in the ring; wait until it has emptied the slot
+  const unsigned long long t0 = wall_clock64();
+  while (__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) {
+    __builtin_amdgcn_s_sleep(1);
+    if (wall_clock64() - t0 > kWaitTimeoutTicks) {
+      atomicOr(&q.sync->error, kErrTimeout);
+      return;
+    }
+  }
+  __hip_atomic_store(slot, ((unsigned long long)pos << 32) | ((unsigned)t + 1u),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pop a tile (called by one thread): take a ticket, then wait on that
+// ticket's own ring slot until the push with the same position fills it.
+// Returns -1 when the grid has converged (nothing queued, nothing
+// running, so no push can come) or after an error. The caller issues an
+// acquire fence before reading the tile's inputs.
+__device__ inline int pop_tile(const TileQueue& q) {
+  Sync* s = q.sync;
+  const unsigned h =
+      __hip_atomic_fetch_add(&s->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long* slot = &q.ring[h % (unsigned)q.ntiles];
+  const unsigned long long t0 = wall_clock64();
+  for (unsigned spin = 0;; spin++) {
+    const unsigned long long v =
+        __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v != 0ull && (unsigned)(v >> 32) == h) {
+      __hip_atomic_store(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return (int)(unsigned)v - 1;
+    }
+    if ((spin & 7u) == 7u) {
+      if (ld_relaxed(&s->pending) == 0u || ld_relaxed(&s->error) != 0u) return -1;
+      if (wall_clock64() - t0 > kWaitTimeoutTicks) {
+        atomicOr(&s->error, kErrTimeout);
+        return -1;
+      }
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Neighbour offsets, indexed like the kEdge* bits below.
+__device__ constexpr int kNbrDx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+__device__ constexpr int kNbrDy[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
+
+// The persistent loop. visit(t, first, nvis) relaxes tile t (all threads
+// of the workgroup call it together). `first` is true on the tile's first
+// visit, when its stored state is not initialised yet; bit k of `nvis` is
+// set when neighbour k (kEdge* order) has been visited, i.e. its stored
+// state may be read, and is meaningful only for neighbours inside the grid.
+// Returns the bits of the neighbours to queue (same value in every thread).
+//
+// The flags are read after the tile's state word is taken (a
+// read-modify-write): a neighbour that published its first visit before
+// that point has pushed us before it too, and we synchronise with that
+// push; one that publishes later pushes us while we run, and we rerun.
+template <class Visit>
+__device__ inline void run_queue(const TileQueue& q, int ntx, int nty, Visit visit) {
+  __shared__ int s_tile;
+  __shared__ int s_first;
+  __shared__ unsigned s_nvis;
+  bool rerun = false;
+  int t = -1;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned long long w0 = wall_clock64();
+      if (!rerun) {
+        t = pop_tile(q);
+        if (t >= 0)
+          __hip_atomic_exchange(&q.state[t], kRunning, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      }
+      s_tile = t;
+      if (t >= 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_first = ld_relaxed(&q.visited[t]) == 0u;
+        const int tx = t % ntx, ty = t / ntx;
+        unsigned nvis = 0;
+        for (int k = 0; k < 8; k++) {
+          const int x = tx + kNbrDx[k], y = ty + kNbrDy[k];
+          if (x >= 0 && x < ntx && y >= 0 && y < nty && ld_relaxed(&q.visited[y * ntx + x]))
+            nvis |= 1u << k;
+        }
+        s_nvis = nvis;
+      }
+      atomicAdd(&q.sync->t_wait, wall_clock64() - w0);
+    }
+    __syncthreads();
+    const int tt = s_tile;
+    const bool first = s_first != 0;
+    const unsigned nvis = s_nvis;
+    if (tt < 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const unsigned bits = visit(tt, first, nvis);
+    // publish the tile's writes before anyone can pop a tile we queue
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (first) {
+        __hip_atomic_store(&q.visited[tt], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the visited flag is ordered before the state words we touch next
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      }
+      const int tx = tt % ntx, ty = tt / ntx;
+      for (int k = 0; k < 8; k++) {
+        if (!(bits & (1u << k))) continue;
+        const int x = tx + kNbrDx[k], y = ty + kNbrDy[k];
+        if (x < 0 || x >= ntx || y < 0 || y >= nty) continue;
+        push_tile(q, y * ntx + x);
+      }
+      // release the tile, or keep it and rerun if a neighbour queued it
+      // while it ran
+      rerun = false;
+      for (;;) {
+        unsigned cur = ld_relaxed(&q.state[tt]);
+        if (cur & kQueued) {
+          __hip_atomic_exchange(&q.state[tt], kRunning, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+          rerun = true;
           break;
         }
-        if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
-          __hip_atomic_fetch_or(&s->error, kErrBarrierTimeout, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
+        if (__hip_atomic_compare_exchange_strong(&q.state[tt], &cur, 0u, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          __hip_atomic_fetch_sub(&q.sync->pending, 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
       }
     }
-    if (ld_relaxed(&s->error) != 0u) ok = 0;
+    __syncthreads();
   }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok != 0;
 }
 
-// Queue tile `t` for round `next` (deduplicated through mark[t]).
-__device__ inline void queue_tile(unsigned* mark, unsigned* list, Sync* s, int ntiles,
-                                  int t, unsigned next) {
-  unsigned old = atomicMax(&mark[t], next);
-  if (old < next) {
-    unsigned pos = atomicAdd(&s->count[next & 1u], 1u);
-    list[(size_t)(next & 1u) * ntiles + pos] = (unsigned)t;
-  }
+// Per-visit counters and phase timers (thread 0; c0..c2: start, staged,
+// relaxed).
+__device__ inline void visit_stats(Sync* s, unsigned it, bool capped, unsigned long long c0,
+                                   unsigned long long c1, unsigned long long c2) {
+  atomicAdd(&s->visits, 1ull);
+  atomicAdd(&s->local_iters, (unsigned long long)it);
+  atomicMax(&s->max_local, it);
+  if (capped) atomicOr(&s->error, kErrLocalCap);
+  const unsigned long long c3 = wall_clock64();
+  atomicAdd(&s->t_stage, c1 - c0);
+  atomicAdd(&s->t_relax, c2 - c1);
+  atomicAdd(&s->t_write, c3 - c2);
 }
 
 // Cross-lane shifts within a wavefront (DPP wave_shr:1 / wave_shl:1).
@@ -118,7 +255,7 @@ __device__ inline float lane_from_above(float v, float edge) {  // lane i <- lan
                                                     0x130, 0xf, 0xf, false));
 }
 
-// Perimeter bits of a tile: which neighbour tiles see a changed cell.
+// Neighbour tiles (bit k of a queue mask).
 enum : unsigned {
   kEdgeN = 1u, kEdgeS = 2u, kEdgeW = 4u, kEdgeE = 8u,
   kEdgeNW = 16u, kEdgeNE = 32u, kEdgeSW = 64u, kEdgeSE = 128u,
@@ -138,88 +275,189 @@ struct TileGeom {
   __device__ static constexpr int at(int r, int c) { return (r + 1) * P + (c + 1); }
 };
 
-// Bits of neighbour tiles affected when tile-local cell (r, c) changes.
-// `diag` adds the corner tiles (8-connected stencils).
-template <class G, bool diag>
-__device__ inline unsigned edge_bits(int r, int c) {
-  unsigned b = 0;
-  b |= (r == 0) ? kEdgeN : 0u;
-  b |= (r == G::TY - 1) ? kEdgeS : 0u;
-  b |= (c == 0) ? kEdgeW : 0u;
-  b |= (c == G::TX - 1) ? kEdgeE : 0u;
-  if (diag) {
-    b |= (r == 0 && c == 0) ? kEdgeNW : 0u;
-    b |= (r == 0 && c == G::TX - 1) ? kEdgeNE : 0u;
-    b |= (r == G::TY - 1 && c == 0) ? kEdgeSW : 0u;
-    b |= (r == G::TY - 1 && c == G::TX - 1) ? kEdgeSE : 0u;
+// ---------------------------------------------------------------- staging
+// Copies the tile + halo of one or two global arrays into LDS (float
+// layout of TileGeom). Cells outside the grid get `oob`. Full-width tiles
+// of grids with nx % 4 == 0 use 128-bit loads; 8 loads per thread and
+// array are in flight before the LDS stores.
+struct StageF32 {
+  const float* p;
+  float* lds;
+  float oob;
+  __device__ float4 load4(size_t g) const { return *reinterpret_cast<const float4*>(p + g); }
+  __device__ float load1(size_t g) const { return p[g]; }
+};
+// uint8 cells staged as floats (bc flags)
+struct StageU8 {
+  const uint8_t* p;
+  float* lds;
+  float oob;
+  __device__ float4 load4(size_t g) const {
+    const uchar4 v = *reinterpret_cast<const uchar4*>(p + g);
+    return make_float4(v.x, v.y, v.z, v.w);
   }
-  return b;
-}
+  __device__ float load1(size_t g) const { return p[g]; }
+};
 
-// Queue the neighbours of tile (tx, ty) named by `bits`.
-__device__ inline void queue_neighbours(unsigned bits, int tx, int ty, int ntx, int nty,
-                                        unsigned* mark, unsigned* list, Sync* s,
-                                        unsigned next) {
-  const int ntiles = ntx * nty;
-  const int dx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
-  const int dy[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
-  for (int k = 0; k < 8; k++) {
-    if (!(bits & (1u << k))) continue;
-    int x = tx + dx[k], y = ty + dy[k];
-    if (x < 0 || x >= ntx || y < 0 || y >= nty) continue;
-    queue_tile(mark, list, s, ntiles, y * ntx + x, next);
-  }
-}
-
-// The persistent round loop. `visit(t, round)` processes tile t (all
-// threads of the workgroup call it together) and queues the neighbours it
-// changed for round + 1. Round 0 visits every tile; round r > 0 the tiles
-// queued during round r - 1. Returns when a round queues nothing.
-template <class Visit>
-__device__ inline void run_rounds(Sync* s, const unsigned* list, int ntiles,
-                                  unsigned max_rounds, Visit visit) {
-  const unsigned nb = gridDim.x;
-  for (unsigned round = 0;; round++) {
-    if (round == 0) {
-      for (int t = blockIdx.x; t < ntiles; t += nb) visit(t, 0u);
-    } else {
-      const unsigned n = ld_relaxed(&s->count[round & 1u]);
-      const unsigned* L = list + (size_t)(round & 1u) * ntiles;
-      for (unsigned k = blockIdx.x; k < n; k += nb) visit((int)L[k], round);
-    }
-    const unsigned consumed = round & 1u;
-    const unsigned long long w0 = wall_clock64();
-    bool ok = grid_barrier(s, nb, [&] {
-      __hip_atomic_store(&s->count[consumed], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    });
-    if (!ok) return;
-    if (threadIdx.x == 0) atomicAdd(&s->t_wait, wall_clock64() - w0);
-    const unsigned next = ld_relaxed(&s->count[(round + 1) & 1u]);
-    if (next == 0u || round + 1 >= max_rounds) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        s->rounds = round + 1;
-        if (next != 0u) atomicOr(&s->error, kErrRoundCap);
+template <class G, class A, class B>
+__device__ inline void stage_tile(const A& a, const B& b, bool two, long long x0, long long y0,
+                                  long long nx, long long ny) {
+  const int tid = threadIdx.x;
+  const bool vec = ((nx & 3) == 0) && (x0 + G::TX <= nx);
+  if (vec) {
+    constexpr int Q = G::TX / 4, NV = G::ROWS * Q, UV = 8;
+    for (int base = 0; base < NV; base += G::THREADS * UV) {
+      float4 va[UV], vb[UV];
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const int r = idx / Q - 1, c4 = idx % Q;
+        const long long gy = y0 + r;
+        va[u] = make_float4(a.oob, a.oob, a.oob, a.oob);
+        vb[u] = make_float4(b.oob, b.oob, b.oob, b.oob);
+        if (idx < NV && gy >= 0 && gy < ny) {
+          const size_t g = (size_t)gy * nx + x0 + 4 * c4;
+          va[u] = a.load4(g);
+          if (two) vb[u] = b.load4(g);
+        }
       }
-      return;
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < NV) {
+          const int r = idx / Q - 1, c = 4 * (idx % Q);
+          const int at = G::at(r, c);
+          a.lds[at] = va[u].x;
+          a.lds[at + 1] = va[u].y;
+          a.lds[at + 2] = va[u].z;
+          a.lds[at + 3] = va[u].w;
+          if (two) {
+            b.lds[at] = vb[u].x;
+            b.lds[at + 1] = vb[u].y;
+            b.lds[at + 2] = vb[u].z;
+            b.lds[at + 3] = vb[u].w;
+          }
+        }
+      }
+    }
+    // halo columns -1 and TX
+    for (int idx = tid; idx < 2 * G::ROWS; idx += G::THREADS) {
+      const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      float fa = a.oob, fb = b.oob;
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+        const size_t g = (size_t)gy * nx + gx;
+        fa = a.load1(g);
+        if (two) fb = b.load1(g);
+      }
+      a.lds[G::at(r, c)] = fa;
+      if (two) b.lds[G::at(r, c)] = fb;
+    }
+  } else {
+    constexpr int E = G::ROWS * (G::TX + 2), U = 8;
+    for (int base = 0; base < E; base += G::THREADS * U) {
+      float fa[U], fb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const int r = idx / (G::TX + 2) - 1, c = idx % (G::TX + 2) - 1;
+        const long long gx = x0 + c, gy = y0 + r;
+        fa[u] = a.oob;
+        fb[u] = b.oob;
+        if (idx < E && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+          const size_t g = (size_t)gy * nx + gx;
+          fa[u] = a.load1(g);
+          if (two) fb[u] = b.load1(g);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < E) {
+          const int r = idx / (G::TX + 2) - 1, c = idx % (G::TX + 2) - 1;
+          a.lds[G::at(r, c)] = fa[u];
+          if (two) b.lds[G::at(r, c)] = fb[u];
+        }
+      }
     }
   }
 }
 
-// Per-visit bookkeeping by thread 0: queue neighbours, counters, phase
-// timers (c0..c3: start, staged, relaxed, written).
-__device__ inline void finish_visit(Sync* s, unsigned* mark, unsigned* list, unsigned bits,
-                                    int tx, int ty, int ntx, int nty, unsigned round,
-                                    unsigned it, bool capped, unsigned long long c0,
-                                    unsigned long long c1, unsigned long long c2) {
-  queue_neighbours(bits, tx, ty, ntx, nty, mark, list, s, round + 1);
-  atomicAdd(&s->visits, 1ull);
-  atomicAdd(&s->local_iters, (unsigned long long)it);
-  atomicMax(&s->max_local, it);
-  if (capped) atomicOr(&s->error, kErrLocalCap);
-  const unsigned long long c3 = wall_clock64();
-  atomicAdd(&s->t_stage, c1 - c0);
-  atomicAdd(&s->t_relax, c2 - c1);
-  atomicAdd(&s->t_write, c3 - c2);
+// Neighbour (kEdge* bit index) owning halo cell (r, c) of a tile.
+template <class G>
+__device__ inline int halo_owner(int r, int c) {
+  if (r < 0) return c < 0 ? 4 : (c >= G::TX ? 5 : 0);
+  if (r >= G::TY) return c < 0 ? 6 : (c >= G::TX ? 7 : 1);
+  return c < 0 ? 2 : 3;
+}
+
+// Calls fn(r, c, from_state) for every halo cell whose owner's state
+// availability differs from what the tile was staged with: on a first
+// visit (staged from the initial values) the halo of visited neighbours
+// (from_state = true); on later visits (staged from the stored state) the
+// halo of neighbours not visited yet (from_state = false). fn must skip
+// cells outside the grid.
+template <class G, class F>
+__device__ inline void for_stale_halo(bool first, unsigned nvis, F fn) {
+  constexpr int H = 2 * (G::TX + 2) + 2 * G::TY;
+  for (int k = threadIdx.x; k < H; k += G::THREADS) {
+    int r, c;
+    if (k < G::TX + 2) {
+      r = -1;
+      c = k - 1;
+    } else if (k < 2 * (G::TX + 2)) {
+      r = G::TY;
+      c = k - (G::TX + 2) - 1;
+    } else {
+      const int m = k - 2 * (G::TX + 2);
+      r = m < G::TY ? m : m - G::TY;
+      c = m < G::TY ? -1 : G::TX;
+    }
+    const bool vis = (nvis >> halo_owner<G>(r, c)) & 1u;
+    if (vis == first) fn(r, c, vis);
+  }
+}
+
+// Neighbour bits for a changed perimeter: compares the saved perimeter
+// (sper: top row, bottom row, left column, right column) with the relaxed
+// tile, word by word. `diag` adds the corner tiles (8-connected).
+template <class G, bool diag, class W>
+__device__ inline unsigned changed_perimeter(const W* sper, const W* lds) {
+  unsigned bits = 0;
+  for (int k = threadIdx.x; k < G::TX; k += G::THREADS) {
+    if (sper[k] != lds[G::at(0, k)])
+      bits |= kEdgeN | (diag && k == 0 ? kEdgeNW : 0u) | (diag && k == G::TX - 1 ? kEdgeNE : 0u);
+    if (sper[G::TX + k] != lds[G::at(G::TY - 1, k)])
+      bits |= kEdgeS | (diag && k == 0 ? kEdgeSW : 0u) | (diag && k == G::TX - 1 ? kEdgeSE : 0u);
+  }
+  for (int k = threadIdx.x; k < G::TY; k += G::THREADS) {
+    if (sper[2 * G::TX + k] != lds[G::at(k, 0)])
+      bits |= kEdgeW | (diag && k == 0 ? kEdgeNW : 0u) | (diag && k == G::TY - 1 ? kEdgeSW : 0u);
+    if (sper[2 * G::TX + G::TY + k] != lds[G::at(k, G::TX - 1)])
+      bits |= kEdgeE | (diag && k == 0 ? kEdgeNE : 0u) | (diag && k == G::TY - 1 ? kEdgeSE : 0u);
+  }
+  return bits;
+}
+
+template <class G, class W>
+__device__ inline void save_perimeter(W* sper, const W* lds) {
+  for (int k = threadIdx.x; k < G::TX; k += G::THREADS) {
+    sper[k] = lds[G::at(0, k)];
+    sper[G::TX + k] = lds[G::at(G::TY - 1, k)];
+  }
+  for (int k = threadIdx.x; k < G::TY; k += G::THREADS) {
+    sper[2 * G::TX + k] = lds[G::at(k, 0)];
+    sper[2 * G::TX + G::TY + k] = lds[G::at(k, G::TX - 1)];
+  }
+}
+
+// OR-reduce a per-thread bit mask over the workgroup (result in all threads).
+__device__ inline unsigned block_or(unsigned bits, unsigned* s_bits) {
+  if (threadIdx.x == 0) *s_bits = 0u;
+  __syncthreads();
+  if (bits) atomicOr(s_bits, bits);
+  __syncthreads();
+  return *s_bits;
 }
 
 }  // namespace tt

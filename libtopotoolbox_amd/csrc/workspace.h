@@ -1,7 +1,7 @@
-// Per-launch device workspace of the persistent relaxation kernels:
-// the control block, the per-tile queue marks and the two round lists.
-// Allocated stream-ordered (hipMallocAsync) and cleared by a kernel, so a
-// launch sequence contains no memset node.
+// Per-launch device workspace of the persistent relaxation kernels: the
+// control block and the tile queue (ring, per-tile state and visited flags).
+// Allocated stream-ordered (hipMallocAsync) and initialised by a kernel, so
+// a launch sequence contains no memset node.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,12 +15,10 @@ __global__ void zero_words(unsigned* p, size_t n);
 
 struct Workspace {
   void* base = nullptr;
-  Sync* sync = nullptr;
-  unsigned* mark = nullptr;
-  unsigned* list = nullptr;
-  size_t words = 0;
+  TileQueue q{};
   hipStream_t stream = nullptr;
 
+  // All ntiles tiles start queued.
   hipError_t alloc(int ntiles, hipStream_t s);
   // Copies the control block into `st` (synchronises the stream) when st
   // is non-null, then releases the memory stream-ordered.
@@ -31,7 +29,7 @@ struct Workspace {
 hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* grid);
 
 // Allocates the workspace, launches `kern` as a persistent grid over
-// `ntiles` tiles (Args must carry mark / list / sync) and releases the
+// `ntiles` tiles (Args must carry a TileQueue `q`) and releases the
 // workspace stream-ordered; fills `stats` (synchronising) when non-null.
 template <class Args>
 hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a,
@@ -42,9 +40,7 @@ hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a
   Workspace ws;
   e = ws.alloc(ntiles, stream);
   if (e != hipSuccess) return e;
-  a.mark = ws.mark;
-  a.list = ws.list;
-  a.sync = ws.sync;
+  a.q = ws.q;
   kern<<<grid, threads, 0, stream>>>(a);
   e = hipGetLastError();
   if (stats) {
