@@ -37,6 +37,9 @@ METRIC = ("fillsinks/excess-topo Gcells/s & %HBM peak, 8192² fp32; "
 HBM_PEAK_GBS = 8000.0
 N = 8192
 BYTES_PER_CELL = {"fill": 9, "fsm": 12}
+# forward + adjoint: fill 9 + spill 4 + (spill, g_out, g_dem) 12; fsm 12 +
+# (u, dem, slopes, g_out) 16 + (g_dem, g_slopes) 8
+BYTES_FWD_ADJ = {"fill": 25, "fsm": 36}
 
 
 def make_inputs(workload: str, n: int):
@@ -92,6 +95,181 @@ def gpu_run(workload: str, arrays: dict, steps: int, warmup: int):
     wall = time.perf_counter() - t0
     per_launch_ms = [a.elapsed_time(b) for a, b in ev]
     return {"wall_s": wall, "launch_ms": per_launch_ms, "stats": st, "out": out}
+
+
+def time_events(fn, steps: int):
+    """Per-call milliseconds of fn() on the current stream (HIP events)."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, [a.elapsed_time(b) for a, b in ev]
+
+
+def fwd_adjoint_run(workload: str, arrays: dict, steps: int, plain_ms: float):
+    """Forward + adjoint at full size (fillsinks: spill forward + scatter;
+    excess topography: forward + lambda relaxation), seed-44 N(0, 1)
+    upstream gradient, against the plain forward's time."""
+    import torch
+
+    from libtopotoolbox_amd import ops
+
+    dev = torch.device("cuda:0")
+    dem = torch.from_numpy(arrays["dem"]).to(dev)
+    g = torch.randn(dem.shape, generator=torch.Generator(device=dev).manual_seed(44),
+                    device=dev)
+    if workload == "fill":
+        bc = torch.from_numpy(arrays["bc"]).to(dev)
+
+        def step():
+            _, spill = ops.fillsinks_spill(dem, bc)
+            return ops.fillsinks_backward(spill, g)
+    else:
+        slopes = torch.from_numpy(arrays["slopes"]).to(dev)
+        out = torch.empty_like(dem)
+
+        def step():
+            ops.excesstopography_fsm2d(dem, slopes, 30.0, out=out)
+            return ops.excesstopography_fsm2d_backward(out, dem, slopes, 30.0, g)
+    step()
+    ms, _ = time_events(step, steps)
+    cells = dem.numel()
+    return {"ms_per_step": round(ms, 4), "gcells_per_s": round(cells / (ms * 1e-3) / 1e9, 4),
+            "plain_forward_ms": round(plain_ms, 4),
+            "fwd_adjoint_over_plain": round(ms / plain_ms, 3),
+            "bytes_per_cell": BYTES_FWD_ADJ[workload]}
+
+
+def batched_inputs(op: str, count: int, seed: int = 5):
+    """512 uneven tiles: side exp(U(ln 128, ln 2048)), aspect U(0.5, 2);
+    per-tile fractal (Hurst 0.8, 0-1000 m, 5% pits) with seed 5000 + t; edge
+    bc (fillsinks) or 8-site Voronoi slopes U(0.2, 0.8) with seed 9000 + t
+    (excess topography); upstream gradient N(0, 1) with seed 7000 + t."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from libtopotoolbox_amd import synthetic as S
+
+    shapes = S.uneven_tile_shapes(count, 128, 2048, seed)
+
+    def one(t):
+        ny, nx = shapes[t]
+        d = S.dem_pits(ny, nx, 5000 + t)
+        g = np.random.default_rng(7000 + t).standard_normal((ny, nx), dtype=np.float32)
+        aux = (S.edge_bc(ny, nx) if op == "fillsinks"
+               else S.voronoi_slopes(ny, nx, 8, 0.2, 0.8, 9000 + t))
+        return d, aux, g
+
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(one, range(count)))
+    return shapes, [r[0] for r in res], [r[1] for r in res], [r[2] for r in res]
+
+
+def batched_run(op: str, count: int, steps: int):
+    """One graph-captured persistent launch over all tiles (forward +
+    adjoint, autotuned tile shape and sweep cap) against the same tiles run
+    one at a time through the plain single-grid entry points."""
+    import torch
+
+    from libtopotoolbox_amd import autotune, ops
+    from libtopotoolbox_amd.batched import Batch
+
+    t0 = time.perf_counter()
+    shapes, dems, aux, gs = batched_inputs(op, count)
+    gen_s = time.perf_counter() - t0
+    cells = sum(a * b for a, b in shapes)
+    b = Batch(op, shapes, cellsize=30.0)
+    if op == "fillsinks":
+        b.load(dems, bc=aux, g_out=gs)
+    else:
+        b.load(dems, slopes=aux, g_out=gs)
+    del dems, aux, gs
+    tune = autotune.tune(b, reps=2, use_cache=False)
+    graph = b.capture(max_local=tune["max_local"])
+    graph.replay()
+    ms, per = time_events(graph.replay, steps)
+    st = b.run(max_local=tune["max_local"], stats=True)
+    fwd_ms, _ = time_events(lambda: b.run(forward_only=True, max_local=tune["max_local"]),
+                            max(2, steps // 2))
+
+    # the same tiles one at a time through the plain kernels
+    dv, ov = b.views(b.dem), b.views(b.output)
+    gv = b.views(b.g_out)
+    if op == "fillsinks":
+        bcv = b.views(b.bc)
+
+        def seq():
+            for d, c, g in zip(dv, bcv, gv):
+                _, spill = ops.fillsinks_spill(d, c)
+                ops.fillsinks_backward(spill, g)
+
+        def seq_fwd():
+            for d, c, o in zip(dv, bcv, ov):
+                ops.fillsinks(d, c)
+    else:
+        sv = b.views(b.slopes)
+
+        def seq():
+            for d, s, g in zip(dv, sv, gv):
+                u = ops.excesstopography_fsm2d(d, s, 30.0)
+                ops.excesstopography_fsm2d_backward(u, d, s, 30.0, g)
+
+        def seq_fwd():
+            for d, s in zip(dv, sv):
+                ops.excesstopography_fsm2d(d, s, 30.0)
+    seq()
+    seq_ms, _ = time_events(seq, 2)
+    seq_fwd_ms, _ = time_events(seq_fwd, 2)
+
+    # spot parity: the batch's results equal the single-grid ones
+    graph.replay()
+    torch.cuda.synchronize()
+    r = b.outputs()
+    worst = 0.0
+    for k in range(0, count, max(1, count // 16)):
+        if op == "fillsinks":
+            o, spill = ops.fillsinks_spill(dv[k], bcv[k])
+            gd = ops.fillsinks_backward(spill, gv[k])
+            same = (torch.equal(o.view(torch.int32), r["output"][k].view(torch.int32))
+                    and torch.equal(spill, r["spill"][k])
+                    and torch.equal(gd.view(torch.int32), r["g_dem"][k].view(torch.int32)))
+            worst = max(worst, 0.0 if same else 1.0)
+        else:
+            u = ops.excesstopography_fsm2d(dv[k], sv[k], 30.0)
+            ok = torch.isfinite(u)
+            rel = ((u - r["output"][k]).abs() / u.abs().clamp_min(1.0))[ok]
+            worst = max(worst, float(rel.max()) if rel.numel() else 0.0)
+    bytes_cell = BYTES_FWD_ADJ["fill" if op == "fillsinks" else "fsm"]
+    launch_ms = float(np.median(per))
+    res = {
+        "tiles": count, "cells": cells, "input_gen_s": round(gen_s, 1),
+        "autotune": {"tile": tune["tile"], "max_local": tune["max_local"],
+                     "table": tune["table"]},
+        "graph_ms": round(ms, 3), "gcells_per_s": round(cells / (ms * 1e-3) / 1e9, 4),
+        "forward_only_ms": round(fwd_ms, 3),
+        "one_at_a_time_ms": round(seq_ms, 3), "one_at_a_time_forward_ms": round(seq_fwd_ms, 3),
+        "one_at_a_time_gcells_per_s": round(cells / (seq_ms * 1e-3) / 1e9, 4),
+        "speedup_vs_one_at_a_time": round(seq_ms / ms, 3),
+        "roofline": {"bound": "hbm", "bytes_per_cell": bytes_cell,
+                     "achieved": round(bytes_cell * cells / (launch_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_cell * cells / (launch_ms * 1e-3) / 1e9
+                                   / HBM_PEAK_GBS, 4)},
+        "kernel_stats": {k: st[k] for k in ("requeues", "visits", "grid", "tile_x", "tile_y")},
+        "parity_spot": ("bitwise" if op == "fillsinks" and worst == 0.0 else
+                        ("MISMATCH" if op == "fillsinks" else f"max rel {worst:.2e}")),
+    }
+    del b, graph
+    torch.cuda.empty_cache()
+    return res
 
 
 def roofline(workload: str, cells: int, launch_ms: float, traffic):
@@ -154,6 +332,9 @@ def main():
     ap.add_argument("--cpu-crop", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-tiles", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -201,6 +382,13 @@ def main():
     del r
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, arrays, min(args.cpu_crop, n))
+    extra = {}
+    if not args.no_adjoint:
+        extra["fwd_adjoint"] = {args.workload: fwd_adjoint_run(args.workload, arrays,
+                                                               max(3, args.steps // 2),
+                                                               launch_ms)}
+        print(f"[bench] fwd+adjoint {args.workload}: {extra['fwd_adjoint'][args.workload]}",
+              file=sys.stderr, flush=True)
     del arrays
     if not args.no_extra:
         a2 = make_inputs(other, n)
@@ -214,7 +402,22 @@ def main():
                                                            "max_local", "grid")}}
         if not args.no_cpu:
             ex["cpu_baseline"] = cpu_baseline(other, a2, min(2048, n))
-        line["extra"] = {other: ex}
+        extra[other] = ex
+        if not args.no_adjoint:
+            extra.setdefault("fwd_adjoint", {})[other] = fwd_adjoint_run(
+                other, a2, max(3, args.steps // 2), float(np.mean(r2["launch_ms"])))
+        del a2, r2
+    if not args.no_batched:
+        import torch
+
+        torch.cuda.empty_cache()
+        extra["batched"] = {}
+        for op in ("fillsinks", "excesstopography"):
+            extra["batched"][op] = batched_run(op, args.batch_tiles, max(3, args.steps // 4))
+            print(f"[bench] batched {op}: {json.dumps(extra['batched'][op])}", file=sys.stderr,
+                  flush=True)
+    if extra:
+        line["extra"] = extra
     print(json.dumps(line), flush=True)
 
 

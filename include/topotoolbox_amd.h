@@ -80,7 +80,7 @@ typedef struct tt_stats {
   int64_t t_stage;     /* staging tiles into LDS */
   int64_t t_relax;     /* local sweeps */
   int64_t t_write;     /* write-back and neighbour queueing */
-  int64_t t_wait;      /* grid-barrier waits */
+  int64_t t_wait;      /* waiting for a tile to pop */
 } tt_stats;
 
 /* fillsinks on device buffers (for callers whose DEM is already in HBM). */
@@ -117,6 +117,63 @@ int tt_excesstopography_fsm2d_backward_dev(float *g_dem, float *g_slopes, const 
                                            const float *dem, const float *slopes, float cellsize,
                                            const float *g_out, ptrdiff_t nx, ptrdiff_t ny,
                                            void *stream, tt_stats *stats);
+
+/* ---------------------------------------------------------------------
+   Batched entry (no reference counterpart: the reference handles one grid
+   per call). Many grids ("items") of arbitrary, uneven sizes run in ONE
+   persistent launch whose tiles share a single work queue; each item goes
+   through the forward pass and, unless TT_BATCH_FORWARD_ONLY, the adjoint,
+   with no host round trip in between. The launch allocates nothing and
+   does not synchronise (unless `stats` is non-NULL), so it can be captured
+   in a HIP graph and replayed.
+
+   Usage: tt_batch_plan once with plan = NULL to get the sizes, again with a
+   host buffer of plan_bytes; copy the plan to device memory; allocate
+   workspace_bytes of device memory; then tt_batch_run_dev as often as
+   wanted (the plan and workspace can be reused by calls that do not
+   overlap in time). Items are packed in the caller's buffers: item b's
+   cell (i, j) is element offset + j * nx + i of every per-cell buffer.
+   Items must not overlap.
+   --------------------------------------------------------------------- */
+
+#define TT_BATCH_FILLSINKS 0         /* forward: fillsinks + spill index; adjoint: g_dem */
+#define TT_BATCH_EXCESSTOPOGRAPHY 1  /* forward: excess; adjoint: g_dem, g_slopes */
+#define TT_BATCH_FORWARD_ONLY 1      /* flag: skip the adjoint phases */
+
+typedef struct tt_batch_item {
+  ptrdiff_t nx;      /* cells along dims[0] */
+  ptrdiff_t ny;      /* cells along dims[1] */
+  ptrdiff_t offset;  /* index of cell (0, 0) in the packed buffers */
+  float cellsize;    /* excess topography only */
+} tt_batch_item;
+
+typedef struct tt_batch_buffers {
+  float *output;          /* filled DEM / excess topography */
+  int32_t *spill;         /* fillsinks: spill index, item-local (as tt_fillsinks_spill_dev) */
+  const float *dem;
+  const uint8_t *bc;      /* fillsinks: boundary cells */
+  const float *slopes;    /* excess topography: threshold slopes */
+  const float *g_out;     /* adjoint seed dL/doutput (unused when forward only) */
+  float *g_dem;           /* dL/ddem */
+  float *g_slopes;        /* excess topography: dL/dthreshold_slopes */
+} tt_batch_buffers;
+
+/* Builds the plan of a batch for tiles of qx x qy 64x64 quadrants
+   (qx * qy <= 2). With plan == NULL only *plan_bytes and *workspace_bytes
+   are set. Host-only: no GPU call. */
+int tt_batch_plan(int op, const tt_batch_item *items, int nitems, int qx, int qy, void *plan,
+                  size_t *plan_bytes, size_t *workspace_bytes);
+
+/* Runs a planned batch. plan_host and plan_dev hold the same plan bytes
+   (host and device copies); `workspace` is device memory of at least
+   workspace_bytes. max_local caps the sweeps of one tile visit (0 = 65536;
+   a capped visit queues its tile again, so the cap changes the schedule,
+   never the result). Results equal the single-grid entry points' bit for
+   bit (fillsinks, spill, the fillsinks adjoint) or to the same FP32
+   rounding behaviour (excess topography and its adjoint). */
+int tt_batch_run_dev(const void *plan_host, const void *plan_dev,
+                     const tt_batch_buffers *buffers, void *workspace, size_t workspace_bytes,
+                     unsigned max_local, int flags, void *stream, tt_stats *stats);
 
 /* Tile shape used by subsequent calls on this thread, in 64x64 quadrants
    (qx, qy in {1, 2}); returns nonzero for an unsupported shape. */

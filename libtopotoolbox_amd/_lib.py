@@ -42,6 +42,36 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchItem(ctypes.Structure):
+    """Mirror of tt_batch_item."""
+
+    _fields_ = [
+        ("nx", ctypes.c_ssize_t),
+        ("ny", ctypes.c_ssize_t),
+        ("offset", ctypes.c_ssize_t),
+        ("cellsize", ctypes.c_float),
+    ]
+
+
+class BatchBuffers(ctypes.Structure):
+    """Mirror of tt_batch_buffers (device pointers)."""
+
+    _fields_ = [
+        ("output", ctypes.c_void_p),
+        ("spill", ctypes.c_void_p),
+        ("dem", ctypes.c_void_p),
+        ("bc", ctypes.c_void_p),
+        ("slopes", ctypes.c_void_p),
+        ("g_out", ctypes.c_void_p),
+        ("g_dem", ctypes.c_void_p),
+        ("g_slopes", ctypes.c_void_p),
+    ]
+
+
+BATCH_FILLSINKS = 0
+BATCH_EXCESSTOPOGRAPHY = 1
+BATCH_FORWARD_ONLY = 1
+
 _vp = ctypes.c_void_p
 _pd = ctypes.c_ssize_t  # ptrdiff_t
 _dims_t = _pd * 2
@@ -62,8 +92,11 @@ SIGNATURES = {
     "tt_excesstopography_fsm2d_backward_dev": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
-    "tt_batch_run_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp,
-                                        ctypes.POINTER(Stats)]),
+    "tt_batch_plan": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t),
+                                     ctypes.POINTER(ctypes.c_size_t)]),
+    "tt_batch_run_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.c_uint,
+                                        ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "tt_last_error": (ctypes.c_char_p, []),
 }

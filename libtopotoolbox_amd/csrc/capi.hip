@@ -35,7 +35,7 @@ int fail(const char* where, hipError_t e) {
 }
 
 int device_error(const char* where, uint32_t bits) {
-  tt::set_error("%s: device error bits 0x%x (1 barrier timeout, 2 round cap, 4 local cap)", where,
+  tt::set_error("%s: device error bits 0x%x (1: a queue wait exceeded its time limit)", where,
                 bits);
   return 1000 + (int)bits;
 }
@@ -186,6 +186,34 @@ int tt_excesstopography_fsm2d_backward_dev(float* g_dem, float* g_slopes, const 
   if (stats) {
     to_public(st, stats);
     if (st.error) return device_error("tt_excesstopography_fsm2d_backward_dev", st.error);
+  }
+  return 0;
+}
+
+int tt_batch_plan(int op, const tt_batch_item* items, int nitems, int qx, int qy, void* plan,
+                  size_t* plan_bytes, size_t* workspace_bytes) {
+  tt::set_error("%s", "");
+  hipError_t e = tt::batch_plan(op, nitems, items, tt::TileShape{qx, qy}, plan, plan_bytes,
+                                workspace_bytes);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int tt_batch_run_dev(const void* plan_host, const void* plan_dev,
+                     const tt_batch_buffers* buffers, void* workspace, size_t workspace_bytes,
+                     unsigned max_local, int flags, void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (!buffers) {
+    tt::set_error("tt_batch_run_dev: buffers is NULL");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_batch(plan_host, plan_dev, *buffers, workspace, workspace_bytes,
+                               max_local, flags, (hipStream_t)stream, stats ? &st : nullptr);
+  if (e == hipErrorInvalidValue && tt::last_error()[0]) return (int)e;
+  if (e != hipSuccess) return fail("tt_batch_run_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_batch_run_dev", st.error);
   }
   return 0;
 }

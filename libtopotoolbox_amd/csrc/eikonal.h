@@ -7,14 +7,18 @@
 namespace tt {
 
 // Proposal for a cell from its x-neighbour minimum u1, y-neighbour minimum
-// u2 and fi = cellsize * slope; the float operations of
-// src/excesstopography.c:36-40 in the same order (built without FMA
-// contraction).
+// u2 and fi = cellsize * slope: the expression of
+// src/excesstopography.c:36-40 with its operations in the same order (the
+// library is built without FMA contraction), evaluated branch-free (both
+// sides, then a select) and with the hardware square root (v_sqrt_f32,
+// within 1 ulp) in place of a correctly rounded one. NaN results (the
+// square root of a negative number when fi is tiny) are never accepted,
+// since proposal < u is false for NaN, as in the reference.
 __device__ __forceinline__ float godunov(float u1, float u2, float fi) {
-  if (fabsf(u1 - u2) < fi) {
-    return (u1 + u2) / 2 + sqrtf(-(u1 - u2) * (u1 - u2) + 2 * (fi * fi)) / 2;
-  }
-  return fminf(u1 + fi, u2 + fi);
+  const float d = u1 - u2;
+  const float two = (u1 + u2) / 2 + __builtin_amdgcn_sqrtf(-d * d + 2 * (fi * fi)) / 2;
+  const float one = fminf(u1 + fi, u2 + fi);
+  return fabsf(d) < fi ? two : one;
 }
 
 }  // namespace tt

@@ -1,0 +1,336 @@
+// Tile visit of the sink-filling / reconstruction relaxation, shared by the
+// single-grid kernels (fill.hip) and the batched kernel (batched.hip).
+//
+// Reference semantics (src/fillsinks.c:22-48, src/reconstruct.c:68-112,
+// 134-202, 270-279): fillsinks negates the DEM, seeds the marker with the
+// negated DEM on bc==1 cells and -INF elsewhere, reconstructs by dilation
+// (marker = min(mask, max(marker, 8 neighbours)) until nothing changes) and
+// negates back. That fixed point is unique and every value is a copy of an
+// input value, so any update order reaches the same bits.
+//
+// The visit works directly in the "fill domain" f = -marker, d = -mask:
+//     f_p <- max(d_p, min(f_p, f_q for the 8 neighbours q))
+// iterated from f0 = d on seeds (+INF elsewhere) down to the greatest fixed
+// point. NaN cells of d are inert: level +INF, bound +INF (fminf skips them
+// as fmaxf skips NaN in the reference), written back as the input NaN.
+//
+// kModeFill:  d = dem, f0 = (bc == 1) ? dem : +INF; the output array holds
+//             f between visits.
+// kModeRecon: d = -mask, f0 = max(d, -marker) (a NaN marker acts as -INF);
+//             the marker array holds -f between visits and at the end.
+// kModeSpill: fillsinks over lexicographic (level, cell index) keys packed
+//             in 64 bits; the index half of the fixed point is the cell
+//             whose DEM value the output copies (the spill cell), which the
+//             adjoint scatters to. Unique fixed point, so deterministic.
+//             Halo reads race with neighbour write-backs, which is harmless
+//             only because every value read is a valid upper bound: hence
+//             one 64-bit key word per cell, never two arrays.
+#pragma once
+
+#include <type_traits>
+
+#include "kernels.h"
+#include "tile_engine.h"
+
+namespace tt {
+
+enum : int { kModeSpill = 2 };
+
+// One grid as seen by a tile visit (spill indices are cell positions in
+// this grid).
+struct FillView {
+  float* state;
+  int* spill;                // kModeSpill only
+  unsigned long long* keys;  // kModeSpill only: the iteration state
+  const float* src;
+  const uint8_t* bc;
+  long long nx, ny;
+};
+
+// ---------------------------------------------------------------- level types
+struct FloatLevel {
+  using T = float;
+  static __device__ __forceinline__ T lo(T a, T b) { return fminf(a, b); }
+  static __device__ __forceinline__ T hi(T a, T b) { return fmaxf(a, b); }
+  static __device__ __forceinline__ T from_below(T v, T e) { return lane_from_below(v, e); }
+  static __device__ __forceinline__ T from_above(T v, T e) { return lane_from_above(v, e); }
+  static constexpr T kUnreached = __builtin_inff();
+  static constexpr T kInert = __builtin_inff();
+};
+
+// Order-preserving map of a float's bits to uint32 (x < y <=> ord(x) < ord(y)
+// for non-NaN x, y, with -0 ordered below +0).
+__device__ __forceinline__ unsigned ord_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u);
+}
+__device__ __forceinline__ float unord_bits(unsigned u) {
+  return __uint_as_float((u >> 31) ? (u ^ 0x80000000u) : ~u);
+}
+
+__device__ __forceinline__ unsigned dpp_u32(unsigned old, unsigned v, int ctrl) {
+  return ctrl == 0x138
+             ? (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x138, 0xf, 0xf, false)
+             : (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+struct KeyLevel {
+  using T = unsigned long long;  // ord(level) << 32 | cell index
+  static __device__ __forceinline__ T lo(T a, T b) { return a < b ? a : b; }
+  static __device__ __forceinline__ T hi(T a, T b) { return a < b ? b : a; }
+  static __device__ __forceinline__ T shift(T v, T e, int ctrl) {
+    const unsigned l = dpp_u32((unsigned)e, (unsigned)v, ctrl);
+    const unsigned h = dpp_u32((unsigned)(e >> 32), (unsigned)(v >> 32), ctrl);
+    return ((T)h << 32) | l;
+  }
+  static __device__ __forceinline__ T from_below(T v, T e) { return shift(v, e, 0x138); }
+  static __device__ __forceinline__ T from_above(T v, T e) { return shift(v, e, 0x130); }
+  // (+INF, no index): a cell no seed has reached yet; all ones: inert bound
+  static constexpr T kUnreached = (0xff800000ull << 32) | 0xffffffffull;
+  static constexpr T kInert = ~0ull;
+  static __device__ __forceinline__ T make(float f, unsigned idx) {
+    return ((T)ord_bits(f) << 32) | idx;
+  }
+};
+
+template <int MODE>
+using FillLevel = typename std::conditional<MODE == kModeSpill, KeyLevel, FloatLevel>::type;
+
+// One directional sweep over a 64x64 quadrant with lanes along columns
+// (vertical sweep: down = +y) or along rows (horizontal: right = +x).
+// The value carried between steps is the lane's previous cell; its two
+// side neighbours come from the adjacent lanes by DPP, with the quadrant's
+// outside column/row read from LDS for the edge lanes. LDS operands are
+// fetched two steps ahead; none of them is written in between (a lane's
+// cell is written only at its own step, the outside line belongs to another
+// wavefront or the halo), and the carried value is stored every step.
+template <class G, class L, bool VERT, bool FWD>
+__device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::T* sd, int R0,
+                                           int C0, int lane) {
+  using T = typename L::T;
+  constexpr int step = FWD ? 1 : -1;
+  bool changed = false;
+  const int own = VERT ? C0 + lane : R0 + lane;      // lane's column / row
+  const int side = (lane == 0) ? own - 1 : own + 1;  // outside neighbour line
+  const int k0 = VERT ? (FWD ? R0 : R0 + 63) : (FWD ? C0 : C0 + 63);
+  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
+  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
+  T pc = sf[cell(k0 - step)];
+  T f0 = sf[cell(k0)], d0 = sd[cell(k0)], e0 = sf[edge_cell(k0 - step)];
+  T f1 = sf[cell(k0 + step)], d1 = sd[cell(k0 + step)], e1 = sf[edge_cell(k0)];
+  int k = k0;
+#pragma unroll 4
+  for (int s = 0; s < 64; s++, k += step) {
+    const T fc = f0, dv = d0, edge = e0;
+    f0 = f1;
+    d0 = d1;
+    e0 = e1;
+    if (s < 62) {
+      f1 = sf[cell(k + 2 * step)];
+      d1 = sd[cell(k + 2 * step)];
+      e1 = sf[edge_cell(k + step)];
+    }
+    const T a = L::from_below(pc, edge);
+    const T b = L::from_above(pc, edge);
+    const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc, fc)));
+    changed |= nv < fc;
+    pc = L::lo(nv, fc);
+    sf[cell(k)] = pc;
+  }
+  return changed;
+}
+
+// Fixed-point test of a wavefront's quadrant: true (in some lane) if a
+// cell would still decrease. Lanes are columns walking the rows with a
+// three-row window in registers; the side columns come by DPP (the
+// quadrant's outside columns from LDS for the edge lanes). The steps are
+// independent, so this costs a fraction of a sweep.
+template <class G, class L>
+__device__ __forceinline__ bool fill_violation(const typename L::T* sf, const typename L::T* sd,
+                                               int R0, int C0, int lane) {
+  using T = typename L::T;
+  const int C = C0 + lane;
+  const int S = (lane == 0) ? C - 1 : C + 1;  // outside column for the edge lanes
+  T up = sf[G::at(R0 - 1, C)], mid = sf[G::at(R0, C)];
+  T eup = sf[G::at(R0 - 1, S)], emid = sf[G::at(R0, S)];
+  bool v = false;
+#pragma unroll 8
+  for (int r = 0; r < 64; r++) {
+    const T dn = sf[G::at(R0 + r + 1, C)];
+    const T edn = sf[G::at(R0 + r + 1, S)];
+    const T dv = sd[G::at(R0 + r, C)];
+    const T col = L::lo(L::lo(up, dn), mid);       // own column, rows r-1..r+1
+    const T ecol = L::lo(L::lo(eup, edn), emid);   // outside column (edge lanes)
+    const T m = L::lo(col, L::lo(L::from_below(col, ecol), L::from_above(col, ecol)));
+    v |= L::hi(dv, m) < mid;
+    up = mid;
+    mid = dn;
+    eup = emid;
+    emid = edn;
+  }
+  return v;
+}
+
+// LDS of one fill visit.
+template <class G, int MODE>
+struct FillLds {
+  using T = typename FillLevel<MODE>::T;
+  T sf[G::CELLS];
+  T sd[G::CELLS];
+  T sper[2 * (G::TX + G::TY)];
+  unsigned bits;
+};
+
+// Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
+// max_local sweep iterations; kSelf is returned if it stopped there) and
+// writes it back. Returns the neighbours to queue. Called by all threads
+// of the workgroup; `sync` receives the visit statistics.
+template <class G, int MODE>
+__device__ inline unsigned fill_visit(const FillView& v, long long x0, long long y0, bool first,
+                                      unsigned nvis, unsigned max_local, Sync* sync,
+                                      FillLds<G, MODE>& lds) {
+  using L = FillLevel<MODE>;
+  using T = typename L::T;
+  T* sf = lds.sf;
+  T* sd = lds.sd;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int R0 = (wave / G::QX) * 64;
+  const int C0 = (wave % G::QX) * 64;
+  const unsigned long long c0 = wall_clock64();
+  // ---- stage tile + halo
+  if constexpr (MODE == kModeSpill) {
+    constexpr int E = G::ROWS * (G::TX + 2);
+    constexpr int U = 8;
+    for (int base = 0; base < E; base += G::THREADS * U) {
+      T fv[U], dv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const int r = idx / (G::TX + 2) - 1;
+        const int c = idx % (G::TX + 2) - 1;
+        const long long gx = x0 + c, gy = y0 + r;
+        T f = L::kUnreached, d = L::kInert;
+        if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + gx;
+          const float raw = v.src[g];
+          if (raw == raw) {
+            d = KeyLevel::make(raw, (unsigned)g);
+            f = first ? ((v.bc[g] == 1) ? d : KeyLevel::kUnreached) : v.keys[g];
+          }
+        }
+        fv[u] = f;
+        dv[u] = d;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < E) {
+          const int r = idx / (G::TX + 2) - 1;
+          const int c = idx % (G::TX + 2) - 1;
+          sf[G::at(r, c)] = fv[u];
+          sd[G::at(r, c)] = dv[u];
+        }
+      }
+    }
+  } else {
+    // raw inputs first (bound source in sd; bc flag or stored state in sf),
+    // then one LDS pass turns them into (level, bound)
+    const StageF32 src{v.src, sd, __builtin_nanf("")};
+    if (MODE == kModeFill && first)
+      stage_tile<G>(src, StageU8{v.bc, sf, 0.f}, true, x0, y0, v.nx, v.ny);
+    else
+      stage_tile<G>(src, StageF32{v.state, sf, kInf}, true, x0, y0, v.nx, v.ny);
+    __syncthreads();
+    for (int k = tid; k < G::CELLS; k += G::THREADS) {
+      float d = sd[k], f = sf[k];
+      if (MODE == kModeFill) {
+        if (first) f = (f == 1.f) ? d : kInf;
+      } else {
+        d = -d;
+        f = fmaxf(d, (f == f) ? -f : kInf);
+      }
+      if (d != d) {  // NaN input: inert cell
+        d = kInf;
+        f = kInf;
+      }
+      sd[k] = d;
+      sf[k] = f;
+    }
+  }
+  __syncthreads();
+  // halo cells whose owner tile's stored state availability differs from
+  // what was staged (recon state is always initialised: the marker)
+  if (MODE != kModeRecon) {
+    for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx < 0 || gx >= v.nx || gy < 0 || gy >= v.ny) return;
+      const size_t g = (size_t)gy * v.nx + gx;
+      const float raw = v.src[g];
+      if (raw != raw) return;  // inert either way
+      if constexpr (MODE == kModeSpill) {
+        sf[G::at(r, c)] =
+            from_state ? v.keys[g]
+                       : ((v.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
+                                         : KeyLevel::kUnreached);
+      } else {
+        sf[G::at(r, c)] = from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf);
+      }
+    });
+    __syncthreads();
+  }
+  save_perimeter<G>(lds.sper, sf);
+  const unsigned long long c1 = wall_clock64();
+  // ---- relax: sweep iterations until the parallel fixed-point test
+  // passes (also tried before the first one)
+  unsigned it = 0;
+  int any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+  int any_change = 0;
+  while (any && it < max_local) {
+    bool ch = fill_sweep<G, L, true, true>(sf, sd, R0, C0, lane);
+    ch |= fill_sweep<G, L, true, false>(sf, sd, R0, C0, lane);
+    ch |= fill_sweep<G, L, false, true>(sf, sd, R0, C0, lane);
+    ch |= fill_sweep<G, L, false, false>(sf, sd, R0, C0, lane);
+    any = __syncthreads_or(ch ? 1 : 0);
+    any_change |= any;
+    it++;
+    if (any) any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+  }
+  const unsigned long long c2 = wall_clock64();
+  // ---- write back: the first visit initialises the tile's state; later
+  // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
+  // input) take the input bits.
+  if (first || any_change) {
+    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
+      const int r = idx / G::TX, c = idx % G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx < v.nx && gy < v.ny) {
+        const size_t g = (size_t)gy * v.nx + gx;
+        const T val = sf[G::at(r, c)];
+        const bool inert = sd[G::at(r, c)] == L::kInert;
+        if constexpr (MODE == kModeSpill) {
+          // one aligned 64-bit store: a neighbour reading its halo
+          // concurrently sees the old or the new key, never a mix
+          v.keys[g] = val;
+          v.state[g] = inert ? v.src[g] : unord_bits((unsigned)(val >> 32));
+          v.spill[g] = inert ? -1 : (int)(unsigned)val;
+        } else {
+          float out = val;
+          if (inert) {
+            const float raw = v.src[g];
+            if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
+          }
+          v.state[g] = (MODE == kModeRecon) ? -out : out;
+        }
+      }
+    }
+  }
+  // ---- changed perimeter -> neighbour tiles to queue
+  unsigned bits = block_or(changed_perimeter<G, true>(lds.sper, sf), &lds.bits);
+  if (any) bits |= kSelf;
+  if (tid == 0) visit_stats(sync, it, c0, c1, c2);
+  return bits;
+}
+
+}  // namespace tt

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/topotoolbox_amd.h"
+
 namespace tt {
 
 struct RunStats {
@@ -52,6 +54,46 @@ hipError_t run_fill_backward(float* g_dem, const int* spill, const float* g_out,
 hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const float* dem,
                             const float* slopes, float cellsize, const float* g_out, int64_t nx,
                             int64_t ny, hipStream_t stream, RunStats* stats);
+
+// ---------------------------------------------------------------- batches
+enum : int {
+  kBatchFill = TT_BATCH_FILLSINKS,
+  kBatchFsm = TT_BATCH_EXCESSTOPOGRAPHY,
+  kBatchForwardOnly = TT_BATCH_FORWARD_ONLY,
+};
+
+using BatchItem = tt_batch_item;
+using BatchBuffers = tt_batch_buffers;
+
+// One item as the kernel sees it (in the plan, device memory).
+struct BatchItemDev {
+  long long nx, ny, offset;
+  float cellsize;
+  int ntx, nty;     // tiles along x, y
+  int ntiles;       // ntx * nty
+  int tile_base;    // global id of the item's first tile
+};
+
+// Plan layout: header | items (BatchItemDev) | tile -> item (unsigned),
+// each part 256-byte aligned. The workspace layout is recorded too.
+constexpr unsigned kBatchPlanMagic = 0x54544250u;  // "PBTT"
+struct BatchPlanHeader {
+  unsigned magic;
+  int op, nitems, ntiles, qx, qy;
+  long long cells;             // end of the packed cell range
+  size_t items_off, tile_item_off, bytes;
+  size_t ws_queue, ws_items, ws_scratch, ws_bytes;
+};
+
+// Host-only: sizes (plan == null) or writes the plan.
+hipError_t batch_plan(int op, int nitems, const BatchItem* items, TileShape ts, void* plan,
+                      size_t* plan_bytes, size_t* workspace_bytes);
+
+// One persistent launch (plus a small init kernel) over a planned batch;
+// no allocation, no synchronisation unless stats is non-null.
+hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuffers& buf,
+                     void* workspace, size_t workspace_bytes, unsigned max_local, int flags,
+                     hipStream_t stream, RunStats* stats);
 
 // Thread-local error message set by the entry points.
 void set_error(const char* fmt, ...);

@@ -13,7 +13,7 @@
 // then the tile's only owner), relaxes it, and pushes each neighbour whose
 // halo it changed. A push onto an idle tile enqueues it; a push onto a
 // running tile only sets QUEUED, and the owner, seeing it when it finishes,
-// relaxes the tile again at once. A counter of queued-or-running tiles
+// appends the tile to the queue. A counter of queued-or-running tiles
 // reaches zero exactly when the grid has converged, and every workgroup
 // then exits. There are no rounds and no grid barriers.
 //
@@ -54,7 +54,6 @@ struct Sync {
 
 enum : unsigned int {
   kErrTimeout = 1u,
-  kErrLocalCap = 4u,
 };
 
 // Upper bound on one wait (wall-clock ticks of the 100 MHz constant
@@ -79,6 +78,14 @@ struct TileQueue {
 
 enum : unsigned { kQueued = 1u, kRunning = 2u };
 
+// Neighbour tiles (bit k of a queue mask); kSelf queues the visited tile
+// itself again (a visit that stopped at its sweep cap before converging).
+enum : unsigned {
+  kEdgeN = 1u, kEdgeS = 2u, kEdgeW = 4u, kEdgeE = 8u,
+  kEdgeNW = 16u, kEdgeNE = 32u, kEdgeSW = 64u, kEdgeSE = 128u,
+  kSelf = 256u,
+};
+
 // Spinning uses relaxed loads only: an acquire load at agent scope
 // invalidates the XCD's L2, which hundreds of spinning workgroups would do
 // continuously. Ordering comes from one fence per visit in run_queue.
@@ -87,13 +94,24 @@ enum : unsigned { kQueued = 1u, kRunning = 2u };
 // the caller's writes). The fetch_or always writes, so the tile's owner or
 // next popper, whose own read-modify-write of the state word follows it,
 // synchronises with it through the fences.
-__device__ inline void push_tile(const TileQueue& q, int t) {
+__device__ inline void enqueue_tile(const TileQueue& q, int t);
+
+// `on_new(t)` runs when the push turns an idle tile into a queued one.
+template <class OnNew>
+__device__ inline void push_tile(const TileQueue& q, int t, OnNew on_new) {
   const unsigned old =
       __hip_atomic_fetch_or(&q.state[t], kQueued, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (old != 0u) return;  // already queued, or running (its owner will rerun it)
+  if (old != 0u) return;  // already queued, or running (its owner requeues it)
   __hip_atomic_fetch_add(&q.sync->pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned pos =
-      __hip_atomic_fetch_add(&q.sync->tail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  on_new(t);
+  enqueue_tile(q, t);
+}
+__device__ inline void push_tile(const TileQueue& q, int t) {
+  push_tile(q, t, [](int) {});
+}
+
+// Write tile t into the ring at push position pos (reserved from tail).
+__device__ inline void fill_slot(const TileQueue& q, unsigned pos, int t) {
   unsigned long long* slot = &q.ring[pos % (unsigned)q.ntiles];
   // the slot's previous entry (pos - ntiles) has a ticket holder, since at
   // most ntiles tiles are in the ring; wait until it has emptied the slot
@@ -107,6 +125,12 @@ __device__ inline void push_tile(const TileQueue& q, int t) {
   }
   __hip_atomic_store(slot, ((unsigned long long)pos << 32) | ((unsigned)t + 1u),
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Append tile t to the ring (its state already says queued).
+__device__ inline void enqueue_tile(const TileQueue& q, int t) {
+  fill_slot(q, __hip_atomic_fetch_add(&q.sync->tail, 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT), t);
 }
 
 // Pop a tile (called by one thread): take a ticket, then wait on that
@@ -138,47 +162,66 @@ __device__ inline int pop_tile(const TileQueue& q) {
   }
 }
 
+// on_release result meaning "nothing to expand".
+constexpr unsigned long long kNoExpand = ~0ull;
+
 // Neighbour offsets, indexed like the kEdge* bits below.
 __device__ constexpr int kNbrDx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
 __device__ constexpr int kNbrDy[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
+
+// Tile topology of one grid of ntx x nty tiles: neighbour k of tile t
+// (kEdge* order), -1 outside; no per-tile bookkeeping.
+struct GridTopo {
+  int ntx, nty;
+  __device__ int neighbour(int t, int k) const {
+    const int x = t % ntx + kNbrDx[k], y = t / ntx + kNbrDy[k];
+    return (x < 0 || x >= ntx || y < 0 || y >= nty) ? -1 : y * ntx + x;
+  }
+  __device__ void on_new(int) const {}
+  __device__ unsigned long long on_release(const TileQueue&, int) const { return kNoExpand; }
+  __device__ void expand(const TileQueue&, unsigned long long) const {}
+};
 
 // The persistent loop. visit(t, first, nvis) relaxes tile t (all threads
 // of the workgroup call it together). `first` is true on the tile's first
 // visit, when its stored state is not initialised yet; bit k of `nvis` is
 // set when neighbour k (kEdge* order) has been visited, i.e. its stored
-// state may be read, and is meaningful only for neighbours inside the grid.
+// state may be read, and is meaningful only for neighbours that exist.
 // Returns the bits of the neighbours to queue (same value in every thread).
 //
 // The flags are read after the tile's state word is taken (a
 // read-modify-write): a neighbour that published its first visit before
 // that point has pushed us before it too, and we synchronise with that
-// push; one that publishes later pushes us while we run, and we rerun.
-template <class Visit>
-__device__ inline void run_queue(const TileQueue& q, int ntx, int nty, Visit visit) {
+// push; one that publishes later pushes us while we run, and the tile is
+// queued again when we finish.
+//
+// Topo supplies neighbour(t, k), on_new(t) (a push made tile t queued),
+// on_release(q, t) (thread 0: tile t went idle; runs before the global
+// pending count drops, so work it counts in keeps the grid alive) and
+// expand(q, token): when on_release returns a token other than kNoExpand,
+// every thread of the workgroup calls expand with it (bulk queueing of
+// tiles that on_release has counted and reserved ring positions for).
+template <class Topo, class Visit>
+__device__ inline void run_queue(const TileQueue& q, const Topo& topo, Visit visit) {
   __shared__ int s_tile;
   __shared__ int s_first;
   __shared__ unsigned s_nvis;
-  bool rerun = false;
-  int t = -1;
+  __shared__ unsigned long long s_expand;
+  auto on_new = [&](int n) { topo.on_new(n); };
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned long long w0 = wall_clock64();
-      if (!rerun) {
-        t = pop_tile(q);
-        if (t >= 0)
-          __hip_atomic_exchange(&q.state[t], kRunning, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-      }
+      const int t = pop_tile(q);
+      if (t >= 0)
+        __hip_atomic_exchange(&q.state[t], kRunning, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_tile = t;
       if (t >= 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_first = ld_relaxed(&q.visited[t]) == 0u;
-        const int tx = t % ntx, ty = t / ntx;
         unsigned nvis = 0;
         for (int k = 0; k < 8; k++) {
-          const int x = tx + kNbrDx[k], y = ty + kNbrDy[k];
-          if (x >= 0 && x < ntx && y >= 0 && y < nty && ld_relaxed(&q.visited[y * ntx + x]))
-            nvis |= 1u << k;
+          const int n = topo.neighbour(t, k);
+          if (n >= 0 && ld_relaxed(&q.visited[n])) nvis |= 1u << k;
         }
         s_nvis = nvis;
       }
@@ -195,31 +238,32 @@ __device__ inline void run_queue(const TileQueue& q, int ntx, int nty, Visit vis
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
+      s_expand = kNoExpand;
       if (first) {
         __hip_atomic_store(&q.visited[tt], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // the visited flag is ordered before the state words we touch next
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       }
-      const int tx = tt % ntx, ty = tt / ntx;
       for (int k = 0; k < 8; k++) {
         if (!(bits & (1u << k))) continue;
-        const int x = tx + kNbrDx[k], y = ty + kNbrDy[k];
-        if (x < 0 || x >= ntx || y < 0 || y >= nty) continue;
-        push_tile(q, y * ntx + x);
+        const int n = topo.neighbour(tt, k);
+        if (n >= 0) push_tile(q, n, on_new);
       }
-      // release the tile, or keep it and rerun if a neighbour queued it
-      // while it ran
-      rerun = false;
+      if (bits & kSelf) push_tile(q, tt, on_new);  // running: requeued when released
+      // release the tile; if a neighbour queued it while it ran, append it
+      // to the queue (it stays outstanding: pending is unchanged), so that
+      // further neighbour updates coalesce into its next visit
       for (;;) {
         unsigned cur = ld_relaxed(&q.state[tt]);
         if (cur & kQueued) {
-          __hip_atomic_exchange(&q.state[tt], kRunning, __ATOMIC_RELAXED,
+          __hip_atomic_exchange(&q.state[tt], kQueued, __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
-          rerun = true;
+          enqueue_tile(q, tt);
           break;
         }
         if (__hip_atomic_compare_exchange_strong(&q.state[tt], &cur, 0u, __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          s_expand = topo.on_release(q, tt);
           __hip_atomic_fetch_sub(&q.sync->pending, 1u, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
           break;
@@ -227,17 +271,18 @@ __device__ inline void run_queue(const TileQueue& q, int ntx, int nty, Visit vis
       }
     }
     __syncthreads();
+    const unsigned long long ex = s_expand;
+    if (ex != kNoExpand) topo.expand(q, ex);
   }
 }
 
 // Per-visit counters and phase timers (thread 0; c0..c2: start, staged,
 // relaxed).
-__device__ inline void visit_stats(Sync* s, unsigned it, bool capped, unsigned long long c0,
+__device__ inline void visit_stats(Sync* s, unsigned it, unsigned long long c0,
                                    unsigned long long c1, unsigned long long c2) {
   atomicAdd(&s->visits, 1ull);
   atomicAdd(&s->local_iters, (unsigned long long)it);
   atomicMax(&s->max_local, it);
-  if (capped) atomicOr(&s->error, kErrLocalCap);
   const unsigned long long c3 = wall_clock64();
   atomicAdd(&s->t_stage, c1 - c0);
   atomicAdd(&s->t_relax, c2 - c1);
@@ -255,11 +300,6 @@ __device__ inline float lane_from_above(float v, float edge) {  // lane i <- lan
                                                     0x130, 0xf, 0xf, false));
 }
 
-// Neighbour tiles (bit k of a queue mask).
-enum : unsigned {
-  kEdgeN = 1u, kEdgeS = 2u, kEdgeW = 4u, kEdgeE = 8u,
-  kEdgeNW = 16u, kEdgeNE = 32u, kEdgeSW = 64u, kEdgeSE = 128u,
-};
 
 // Tile geometry: QX x QY quadrants of 64x64 cells, one wavefront each.
 // "row" = y = dims[1] index, "col" = x = dims[0] index.
@@ -278,7 +318,8 @@ struct TileGeom {
 // ---------------------------------------------------------------- staging
 // Copies the tile + halo of one or two global arrays into LDS (float
 // layout of TileGeom). Cells outside the grid get `oob`. Full-width tiles
-// of grids with nx % 4 == 0 use 128-bit loads; 8 loads per thread and
+// of grids with nx % 4 == 0 and aligned base pointers use 128-bit loads
+// (uint8: 32-bit); 8 loads per thread and
 // array are in flight before the LDS stores.
 struct StageF32 {
   const float* p;
@@ -286,6 +327,7 @@ struct StageF32 {
   float oob;
   __device__ float4 load4(size_t g) const { return *reinterpret_cast<const float4*>(p + g); }
   __device__ float load1(size_t g) const { return p[g]; }
+  __device__ bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 };
 // uint8 cells staged as floats (bc flags)
 struct StageU8 {
@@ -297,13 +339,14 @@ struct StageU8 {
     return make_float4(v.x, v.y, v.z, v.w);
   }
   __device__ float load1(size_t g) const { return p[g]; }
+  __device__ bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 3) == 0; }
 };
 
 template <class G, class A, class B>
 __device__ inline void stage_tile(const A& a, const B& b, bool two, long long x0, long long y0,
                                   long long nx, long long ny) {
   const int tid = threadIdx.x;
-  const bool vec = ((nx & 3) == 0) && (x0 + G::TX <= nx);
+  const bool vec = ((nx & 3) == 0) && (x0 + G::TX <= nx) && a.aligned() && (!two || b.aligned());
   if (vec) {
     constexpr int Q = G::TX / 4, NV = G::ROWS * Q, UV = 8;
     for (int base = 0; base < NV; base += G::THREADS * UV) {

@@ -40,7 +40,7 @@ def load_golden(name):
 
 def bits(a):
     a = np.ascontiguousarray(a)
-    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+    return a.view({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[a.dtype.itemsize])
 
 
 def assert_bitwise(a, b, what=""):
