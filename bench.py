@@ -335,7 +335,21 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-tiles", type=int, default=512)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB, 5 times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, not a measured path", file=sys.stderr)
@@ -383,6 +397,21 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, arrays, min(args.cpu_crop, n))
     extra = {}
+    # the host-pointer drop-in (reference signature): PCIe copies included
+    from libtopotoolbox_amd import ops
+
+    host_in = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+    fn = (ops.fillsinks_host if args.workload == "fill" else
+          (lambda dem, slopes: ops.excesstopography_fsm2d_host(dem, slopes, 30.0)))
+    fn(*host_in.values())
+    t0 = time.perf_counter()
+    fn(*host_in.values())
+    hd_ms = (time.perf_counter() - t0) * 1e3
+    extra["host_dropin"] = {"ms": round(hd_ms, 3),
+                            "gcells_per_s": round(cells / (hd_ms * 1e-3) / 1e9, 4),
+                            "note": "reference-signature call on host buffers, H2D + kernels "
+                                    "+ D2H included"}
+    del host_in
     if not args.no_adjoint:
         extra["fwd_adjoint"] = {args.workload: fwd_adjoint_run(args.workload, arrays,
                                                                max(3, args.steps // 2),
