@@ -14,6 +14,7 @@ from .ops import (
     fillsinks_host,
     reconstruct,
     reconstruct_host,
+    set_fill_method,
     set_tile_shape,
 )
 
@@ -24,6 +25,7 @@ __all__ = [
     "fillsinks_host",
     "reconstruct",
     "reconstruct_host",
+    "set_fill_method",
     "set_tile_shape",
 ]
 

@@ -98,6 +98,8 @@ SIGNATURES = {
     "tt_batch_run_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.c_uint,
                                         ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "tt_set_fill_method": (ctypes.c_int, [ctypes.c_int]),
+    "tt_set_fill_edge_cap": (ctypes.c_int, [ctypes.c_int]),
     "tt_last_error": (ctypes.c_char_p, []),
 }
 

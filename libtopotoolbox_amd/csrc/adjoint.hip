@@ -166,7 +166,7 @@ hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const
     a.nty = (int)((ny + G::TY - 1) / G::TY);
     a.max_local = 1u << 16;
     e = launch_persistent(fsm_adjoint_persistent<G>, G::THREADS, a.ntx * a.nty, a, stream, stats,
-                          G::TX, G::TY);
+                          G::TX, G::TY, a.ntx, a.nty);
   }
   if (e == hipSuccess) {
     fsm_adjoint_finish<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, g_slopes, lam, dfic, n);

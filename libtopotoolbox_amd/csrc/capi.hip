@@ -12,7 +12,13 @@ const char* last_error();
 
 namespace {
 
+// Tile shape set by tt_set_tile_shape; until then each entry point uses
+// the shape measured fastest for it at 8192^2 on MI355X.
 thread_local tt::TileShape g_shape;
+thread_local bool g_shape_set = false;
+
+tt::TileShape shape_for(tt::TileShape fallback) { return g_shape_set ? g_shape : fallback; }
+thread_local int g_fill_method = tt::kFillQueue;
 
 void to_public(const tt::RunStats& s, tt_stats* o) {
   o->requeues = s.requeues;
@@ -82,6 +88,24 @@ int tt_set_tile_shape(int qx, int qy) {
     return 1;
   }
   g_shape = s;
+  g_shape_set = true;
+  return 0;
+}
+
+int tt_set_fill_method(int method) {
+  if (method != tt::kFillGraph && method != tt::kFillQueue) {
+    tt::set_error("tt_set_fill_method: unknown method %d", method);
+    return 1;
+  }
+  g_fill_method = method;
+  return 0;
+}
+
+int tt_set_fill_edge_cap(int cap) {
+  if (!tt::set_graph_edge_cap(cap)) {
+    tt::set_error("tt_set_fill_edge_cap: cap must be in 1..2048");
+    return 1;
+  }
   return 0;
 }
 
@@ -93,7 +117,8 @@ int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff
     return 1;
   }
   tt::RunStats st;
-  hipError_t e = tt::run_fill(output, dem, bc, nx, ny, tt::kModeFill, g_shape,
+  hipError_t e = tt::run_fill(output, dem, bc, nx, ny, tt::kModeFill, shape_for({2, 2}),
+                              g_fill_method,
                               (hipStream_t)stream, stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_fillsinks_dev", e);
   if (stats) {
@@ -111,8 +136,8 @@ int tt_reconstruct_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t
     return 1;
   }
   tt::RunStats st;
-  hipError_t e = tt::run_fill(marker, mask, nullptr, nx, ny, tt::kModeRecon, g_shape,
-                              (hipStream_t)stream, stats ? &st : nullptr);
+  hipError_t e = tt::run_fill(marker, mask, nullptr, nx, ny, tt::kModeRecon, shape_for({2, 2}),
+                              g_fill_method, (hipStream_t)stream, stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_reconstruct_dev", e);
   if (stats) {
     to_public(st, stats);
@@ -130,7 +155,8 @@ int tt_excesstopography_fsm2d_dev(float* excess, const float* dem, const float* 
     return 1;
   }
   tt::RunStats st;
-  hipError_t e = tt::run_fsm(excess, dem, slopes, cellsize, nx, ny, g_shape, (hipStream_t)stream,
+  hipError_t e = tt::run_fsm(excess, dem, slopes, cellsize, nx, ny, shape_for({2, 1}),
+                             (hipStream_t)stream,
                              stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_excesstopography_fsm2d_dev", e);
   if (stats) {
@@ -148,7 +174,8 @@ int tt_fillsinks_spill_dev(float* output, int32_t* spill, const float* dem, cons
     return 1;
   }
   tt::RunStats st;
-  hipError_t e = tt::run_fill_spill(output, spill, dem, bc, nx, ny, g_shape, (hipStream_t)stream,
+  hipError_t e = tt::run_fill_spill(output, spill, dem, bc, nx, ny, shape_for({2, 1}),
+                                    (hipStream_t)stream,
                                     stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_fillsinks_spill_dev", e);
   if (stats) {

@@ -10,11 +10,16 @@ struct FillArgs {
   int ntx, nty;
   TileQueue q;
   unsigned max_local;
+  const unsigned* gate;  // if set: run only when *gate has one of gate_bits
+  unsigned gate_bits;
 };
 
 template <class G, int MODE>
 __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
   __shared__ FillLds<G, MODE> lds;
+  if (a.gate && !(__hip_atomic_load(a.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                  a.gate_bits))
+    return;
   run_queue(a.q, GridTopo{a.ntx, a.nty}, [&](int t, bool first, unsigned nvis) {
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
     return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.sync, lds);
@@ -27,7 +32,7 @@ static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats) {
   a.nty = (int)((a.v.ny + G::TY - 1) / G::TY);
   a.max_local = 1u << 16;
   return launch_persistent(fill_persistent<G, MODE>, G::THREADS, a.ntx * a.nty, a, stream, stats,
-                           G::TX, G::TY);
+                           G::TX, G::TY, a.ntx, a.nty);
 }
 
 template <int MODE>
@@ -45,8 +50,9 @@ static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, Ru
   }
 }
 
-hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
-                    int mode, TileShape ts, hipStream_t stream, RunStats* stats) {
+hipError_t run_fill_gated(float* state, const float* src, const uint8_t* bc, int64_t nx,
+                          int64_t ny, int mode, TileShape ts, hipStream_t stream,
+                          RunStats* stats, const unsigned* gate, unsigned gate_bits) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
   FillArgs a{};
   a.v.state = state;
@@ -54,8 +60,17 @@ hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t n
   a.v.bc = bc;
   a.v.nx = nx;
   a.v.ny = ny;
+  a.gate = gate;
+  a.gate_bits = gate_bits;
   if (mode == kModeFill) return dispatch_fill<kModeFill>(a, ts, stream, stats);
   return dispatch_fill<kModeRecon>(a, ts, stream, stats);
+}
+
+hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
+                    int mode, TileShape ts, int method, hipStream_t stream, RunStats* stats) {
+  if (method == kFillGraph && mode == kModeFill)
+    return run_fill_graph(state, src, bc, nx, ny, ts, stream, stats);
+  return run_fill_gated(state, src, bc, nx, ny, mode, ts, stream, stats, nullptr, 0u);
 }
 
 hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc,

@@ -52,6 +52,7 @@ struct FloatLevel {
   using T = float;
   static __device__ __forceinline__ T lo(T a, T b) { return fminf(a, b); }
   static __device__ __forceinline__ T hi(T a, T b) { return fmaxf(a, b); }
+  static __device__ __forceinline__ bool less(T a, T b) { return a < b; }
   static __device__ __forceinline__ T from_below(T v, T e) { return lane_from_below(v, e); }
   static __device__ __forceinline__ T from_above(T v, T e) { return lane_from_above(v, e); }
   static constexpr T kUnreached = __builtin_inff();
@@ -78,6 +79,7 @@ struct KeyLevel {
   using T = unsigned long long;  // ord(level) << 32 | cell index
   static __device__ __forceinline__ T lo(T a, T b) { return a < b ? a : b; }
   static __device__ __forceinline__ T hi(T a, T b) { return a < b ? b : a; }
+  static __device__ __forceinline__ bool less(T a, T b) { return a < b; }
   static __device__ __forceinline__ T shift(T v, T e, int ctrl) {
     const unsigned l = dpp_u32((unsigned)e, (unsigned)v, ctrl);
     const unsigned h = dpp_u32((unsigned)(e >> 32), (unsigned)(v >> 32), ctrl);
@@ -133,7 +135,7 @@ __device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::
     const T a = L::from_below(pc, edge);
     const T b = L::from_above(pc, edge);
     const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc, fc)));
-    changed |= nv < fc;
+    changed |= L::less(nv, fc);
     pc = L::lo(nv, fc);
     sf[cell(k)] = pc;
   }
@@ -162,7 +164,7 @@ __device__ __forceinline__ bool fill_violation(const typename L::T* sf, const ty
     const T col = L::lo(L::lo(up, dn), mid);       // own column, rows r-1..r+1
     const T ecol = L::lo(L::lo(eup, edn), emid);   // outside column (edge lanes)
     const T m = L::lo(col, L::lo(L::from_below(col, ecol), L::from_above(col, ecol)));
-    v |= L::hi(dv, m) < mid;
+    v |= L::less(L::hi(dv, m), mid);
     up = mid;
     mid = dn;
     eup = emid;
@@ -326,8 +328,13 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       }
     }
   }
-  // ---- changed perimeter -> neighbour tiles to queue
-  unsigned bits = block_or(changed_perimeter<G, true>(lds.sper, sf), &lds.bits);
+  // ---- changed perimeter -> neighbour tiles whose cells it may lower
+  unsigned bits = block_or(pushes_from_perimeter<G, true>(lds.sper, sf,
+                                                          [&](int p, int q) {
+                                                            return L::less(L::hi(sd[q], sf[p]),
+                                                                           sf[q]);
+                                                          }),
+                           &lds.bits);
   if (any) bits |= kSelf;
   if (tid == 0) visit_stats(sync, it, c0, c1, c2);
   return bits;

@@ -27,7 +27,7 @@ static hipError_t launch_fsm(FsmArgs a, hipStream_t stream, RunStats* stats) {
   a.nty = (int)((a.v.ny + G::TY - 1) / G::TY);
   a.max_local = 1u << 16;
   return launch_persistent(fsm_persistent<G>, G::THREADS, a.ntx * a.nty, a, stream, stats, G::TX,
-                           G::TY);
+                           G::TY, a.ntx, a.nty);
 }
 
 hipError_t run_fsm(float* u, const float* dem, const float* slopes, float cellsize, int64_t nx,

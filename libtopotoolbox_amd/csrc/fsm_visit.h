@@ -172,8 +172,12 @@ __device__ inline unsigned fsm_visit(const FsmView& v, long long x0, long long y
       if (gx < v.nx && gy < v.ny) v.u[(size_t)gy * v.nx + gx] = su[G::at(r, c)];
     }
   }
-  // NaN cells compare as words, so they never count as changed
-  unsigned bits = block_or(changed_perimeter<G, false>(lds.sper, su_bits), &lds.bits);
+  // NaN cells compare as words, so they never count as changed. Every
+  // Godunov proposal through p is at least u_p, so a neighbour cell q can
+  // only be lowered by p if u_p < u_q.
+  unsigned bits = block_or(
+      pushes_from_perimeter<G, false>(lds.sper, su_bits, [&](int p, int q) { return su[p] < su[q]; }),
+      &lds.bits);
   if (any) bits |= kSelf;
   if (tid == 0) visit_stats(sync, it, c0, c1, c2);
   return bits;

@@ -1,5 +1,6 @@
 // Workspace, occupancy and error plumbing shared by the kernels.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -13,8 +14,43 @@ __global__ void zero_words(unsigned* p, size_t n) {
   for (; i < n; i += stride) p[i] = 0u;
 }
 
+// Position of tile t in the initial queue when tiles are ordered from the
+// grid boundary inwards (concentric rings of tiles, each ring in row-major
+// order): a tile's first visit then tends to find its outer neighbours,
+// through which drainage from the edge arrives, already relaxed.
+__device__ inline int boundary_first_tile(int k, int ntx, int nty) {
+  // ring r holds the tiles whose distance to the boundary is r; count the
+  // tiles of the rings before k's ring by peeling rings off the rectangle
+  int r = 0, w = ntx, h = nty;
+  for (;;) {
+    const int n = (w <= 0 || h <= 0) ? 0 : ((w == 1 || h == 1) ? w * h : 2 * w + 2 * h - 4);
+    if (k < n || n == 0) break;
+    k -= n;
+    r++;
+    w -= 2;
+    h -= 2;
+  }
+  // k-th tile (row-major) of ring r of the w x h rectangle at offset (r, r)
+  int tx, ty;
+  if (h == 1 || w == 1) {
+    tx = (h == 1) ? k : 0;
+    ty = (h == 1) ? 0 : k;
+  } else if (k < w) {  // top row
+    tx = k;
+    ty = 0;
+  } else if (k < w + 2 * (h - 2)) {  // side columns, row by row
+    const int m = k - w;
+    ty = 1 + m / 2;
+    tx = (m % 2) ? w - 1 : 0;
+  } else {  // bottom row
+    tx = k - w - 2 * (h - 2);
+    ty = h - 1;
+  }
+  return (ty + r) * ntx + (tx + r);
+}
+
 __global__ void init_queue(Sync* s, unsigned long long* ring, unsigned* state,
-                           unsigned* visited, int ntiles) {
+                           unsigned* visited, int ntiles, int ntx, int nty) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     unsigned* w = reinterpret_cast<unsigned*>(s);
@@ -22,14 +58,18 @@ __global__ void init_queue(Sync* s, unsigned long long* ring, unsigned* state,
     s->tail = (unsigned)ntiles;
     s->pending = (unsigned)ntiles;
   }
+  const bool rings = ntx > 0 && nty > 0 && ntx * nty == ntiles;
   for (int k = i; k < ntiles; k += gridDim.x * blockDim.x) {
-    ring[k] = ((unsigned long long)k << 32) | ((unsigned)k + 1u);
+    const int t = rings ? boundary_first_tile(k, ntx, nty) : k;
+    ring[k] = ((unsigned long long)k << 32) | ((unsigned)t + 1u);
     state[k] = kQueued;
     visited[k] = 0u;
   }
 }
 
-hipError_t Workspace::alloc(int ntiles, hipStream_t s) {
+hipError_t Workspace::alloc(int ntiles, hipStream_t s) { return alloc(ntiles, 0, 0, s); }
+
+hipError_t Workspace::alloc(int ntiles, int ntx, int nty, hipStream_t s) {
   stream = s;
   // Sync block (rounded to 64 words) + ring (2 words per tile) + state and
   // visited (1 word per tile)
@@ -45,7 +85,13 @@ hipError_t Workspace::alloc(int ntiles, hipStream_t s) {
   int blocks = (ntiles + 255) / 256;
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
-  init_queue<<<blocks, 256, 0, s>>>(q.sync, q.ring, q.state, q.visited, ntiles);
+  // TT_QUEUE_ROWMAJOR=1 keeps index order (diagnostics: compares orders)
+  static const bool rowmajor = [] {
+    const char* v = getenv("TT_QUEUE_ROWMAJOR");
+    return v && v[0] == '1';
+  }();
+  if (rowmajor) ntx = nty = 0;
+  init_queue<<<blocks, 256, 0, s>>>(q.sync, q.ring, q.state, q.visited, ntiles, ntx, nty);
   return hipGetLastError();
 }
 

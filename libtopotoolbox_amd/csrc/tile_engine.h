@@ -482,6 +482,54 @@ __device__ inline unsigned changed_perimeter(const W* sper, const W* lds) {
   return bits;
 }
 
+// Neighbour bits for a relaxed tile, selective version: for every
+// perimeter cell p whose value changed (lds vs the saved sper), each halo
+// cell q adjacent to p (8- or 4-connected) is checked with lowers(p, q)
+// (LDS indices); only the owners of cells that the new value of p may
+// still lower are queued. The halo holds the neighbours' values as
+// staged, which can only be higher than their current ones, so "cannot
+// lower the staged value" implies "cannot lower the current value".
+template <class G, bool diag, class W, class Lowers>
+__device__ inline unsigned pushes_from_perimeter(const W* sper, const W* lds, Lowers lowers) {
+  unsigned bits = 0;
+  auto check = [&](int pr, int pc, int qr, int qc) {
+    if (lowers(G::at(pr, pc), G::at(qr, qc))) bits |= 1u << halo_owner<G>(qr, qc);
+  };
+  for (int k = threadIdx.x; k < G::TX; k += G::THREADS) {
+    if (sper[k] != lds[G::at(0, k)]) {
+      check(0, k, -1, k);
+      if (diag) {
+        check(0, k, -1, k - 1);
+        check(0, k, -1, k + 1);
+      }
+    }
+    if (sper[G::TX + k] != lds[G::at(G::TY - 1, k)]) {
+      check(G::TY - 1, k, G::TY, k);
+      if (diag) {
+        check(G::TY - 1, k, G::TY, k - 1);
+        check(G::TY - 1, k, G::TY, k + 1);
+      }
+    }
+  }
+  for (int k = threadIdx.x; k < G::TY; k += G::THREADS) {
+    if (sper[2 * G::TX + k] != lds[G::at(k, 0)]) {
+      check(k, 0, k, -1);
+      if (diag) {
+        check(k, 0, k - 1, -1);
+        check(k, 0, k + 1, -1);
+      }
+    }
+    if (sper[2 * G::TX + G::TY + k] != lds[G::at(k, G::TX - 1)]) {
+      check(k, G::TX - 1, k, G::TX);
+      if (diag) {
+        check(k, G::TX - 1, k - 1, G::TX);
+        check(k, G::TX - 1, k + 1, G::TX);
+      }
+    }
+  }
+  return bits;
+}
+
 template <class G, class W>
 __device__ inline void save_perimeter(W* sper, const W* lds) {
   for (int k = threadIdx.x; k < G::TX; k += G::THREADS) {

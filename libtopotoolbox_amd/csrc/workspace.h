@@ -18,8 +18,10 @@ struct Workspace {
   TileQueue q{};
   hipStream_t stream = nullptr;
 
-  // All ntiles tiles start queued.
+  // All ntiles tiles start queued, in index order.
   hipError_t alloc(int ntiles, hipStream_t s);
+  // All tiles of an ntx x nty grid start queued, boundary rings first.
+  hipError_t alloc(int ntiles, int ntx, int nty, hipStream_t s);
   // Copies the control block into `st` (synchronises the stream) when st
   // is non-null, then releases the memory stream-ordered.
   hipError_t finish(RunStats* st);
@@ -31,14 +33,16 @@ hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* gri
 // Allocates the workspace, launches `kern` as a persistent grid over
 // `ntiles` tiles (Args must carry a TileQueue `q`) and releases the
 // workspace stream-ordered; fills `stats` (synchronising) when non-null.
+// Tiles of an ntx x nty grid start in boundary-first order.
 template <class Args>
 hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a,
-                             hipStream_t stream, RunStats* stats, int tile_x, int tile_y) {
+                             hipStream_t stream, RunStats* stats, int tile_x, int tile_y,
+                             int ntx = 0, int nty = 0) {
   int grid = 0;
   hipError_t e = persistent_grid((const void*)kern, threads, ntiles, &grid);
   if (e != hipSuccess) return e;
   Workspace ws;
-  e = ws.alloc(ntiles, stream);
+  e = ws.alloc(ntiles, ntx, nty, stream);
   if (e != hipSuccess) return e;
   a.q = ws.q;
   kern<<<grid, threads, 0, stream>>>(a);

@@ -110,6 +110,48 @@ def test_fillsinks_pits_1024(cuda, oracle_lib):
     print("stats", st)
 
 
+@pytest.mark.parametrize("method,cap", [("queue", 2048), ("graph", 2048), ("graph", 3)])
+def test_fill_methods(cuda, oracle_lib, method, cap):
+    """Both fillsinks algorithms (tile graph, work queue) and the tile
+    graph's on-device fallback (an edge table of 3 forces it) give the
+    oracle's bits over every tile shape; reconstruct (always the work
+    queue) is unaffected by the setting."""
+    from libtopotoolbox_amd import _lib, ops
+
+    ops.set_fill_method(method)
+    _lib.check(_lib.lib().tt_set_fill_edge_cap(cap), "tt_set_fill_edge_cap")
+    try:
+        rng = np.random.default_rng(17)
+        for shape in [(1, 1), (3, 200), (129, 1), (65, 129), (200, 257)]:
+            dem = oracle_lib.grid_random(shape[0] * 7 + shape[1], shape)
+            dem[rng.uniform(size=shape) < 0.03] = np.nan
+            if shape[0] > 100:
+                dem = np.round(dem * 4) / 4  # flats and ties
+            bc = edge_bc(shape) | (rng.uniform(size=shape) < 0.01).astype(np.uint8)
+            want = oracle_lib.fillsinks_pf(dem, bc)
+            for tile in TILES:
+                out, _ = gpu_fill(dem, bc, tile)
+                assert_bitwise(out, want, f"{method} cap {cap} shape {shape} tile {tile}")
+        mask = oracle_lib.grid_random(78, (150, 190))
+        mask[rng.uniform(size=mask.shape) < 0.03] = np.nan
+        marker = np.where(rng.uniform(size=mask.shape) < 0.05, mask + 5.0,
+                          -np.inf).astype(np.float32)
+        want, _, conv = oracle_lib.reconstruct(marker, mask)
+        assert conv
+        import libtopotoolbox_amd as tt
+
+        for tile in TILES:
+            tt.set_tile_shape(*tile)
+            out = tt.reconstruct(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda())
+            assert_bitwise(out.cpu().numpy(), want, f"{method} cap {cap} reconstruct {tile}")
+    finally:
+        ops.set_fill_method("queue")
+        _lib.lib().tt_set_fill_edge_cap(2048)
+        import libtopotoolbox_amd as tt
+
+        tt.set_tile_shape(2, 2)
+
+
 def test_reconstruct(cuda, oracle_lib):
     import libtopotoolbox_amd as tt
 
