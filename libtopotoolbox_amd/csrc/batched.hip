@@ -8,16 +8,18 @@
 // through phases; its next phase is queued, all its tiles at once, when the
 // last tile of the current phase is released (a per-item pending count):
 //
-//   fillsinks:        0 spill-key relaxation (fill_visit, kModeSpill)
-//                     1 fixed-point accumulator := own contribution
-//                     2 scatter of g_out onto spill cells
-//                     3 g_dem from the accumulator
+//   fillsinks:        0 fill levels (fill_visit, kModeFill)
+//                     1 spill keys from the levels (fill_visit, kModeSpill)
+//                     2 fixed-point accumulator := own contribution
+//                     3 scatter of g_out onto spill cells
+//                     4 g_dem from the accumulator
 //   excesstopography: 0 relaxation of u (fsm_visit)
 //                     1 linearisation (sx, sy, dfic)
 //                     2 relaxation of lambda (adj_visit)
 //                     3 g_dem, g_slopes
 //
-// Forward-only runs stop after phase 0. The launch allocates nothing and
+// Forward-only runs stop after the forward phases (fillsinks: 1,
+// excess topography: 0). The launch allocates nothing and
 // never synchronises: the plan (item table) and the workspace come from
 // the caller, so the call can be captured in a graph.
 #include <stdio.h>
@@ -40,6 +42,7 @@ struct BatchLds;
 template <class G>
 struct BatchLds<G, kBatchFill> {
   union {
+    FillLds<G, kModeFill> level;
     FillLds<G, kModeSpill> fill;
     struct {
       int keys[1024];
@@ -189,17 +192,20 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
     };
     if constexpr (OP == kBatchFill) {
       unsigned long long* acc = a.acc + off;
-      if (ph == 0) {
+      if (ph <= 1) {
         const FillView v{a.buf.output + off, a.buf.spill + off, acc, a.buf.dem + off,
-                         a.buf.bc + off, nx, ny};
+                         a.buf.bc + off, nx, ny, ph == 1};
+        if (ph == 0)
+          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, a.max_local, a.q.sync,
+                                          lds.level);
         return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, a.max_local, a.q.sync,
                                          lds.fill);
       }
       const int* spill = a.buf.spill + off;
       const float* g = a.buf.g_out + off;
-      if (ph == 1) {
+      if (ph == 2) {
         for_cells([&](size_t p) { acc[p] = (spill[p] == (int)p) ? to_fixed(g[p]) : 0ull; });
-      } else if (ph == 2) {
+      } else if (ph == 3) {
         scatter_tile<G>(acc, spill, g, nx, ny, x0, y0, lds.hash.keys, lds.hash.vals);
       } else {
         float* g_dem = a.buf.g_dem + off;
@@ -426,7 +432,7 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
   a.item_pending = a.item_phase + h.nitems;
   a.nitems = h.nitems;
   a.ntiles = h.ntiles;
-  a.last_phase = fwd_only ? 0 : 3;
+  a.last_phase = (h.op == kBatchFill) ? (fwd_only ? 1 : 4) : (fwd_only ? 0 : 3);
   a.buf = buf;
   a.max_local = max_local ? max_local : (1u << 16);
   a.q.sync = reinterpret_cast<Sync*>(ws + h.ws_queue);

@@ -84,9 +84,15 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   a.v.bc = bc;
   a.v.nx = nx;
   a.v.ny = ny;
+  // exact levels first (float relaxation, the fast path), then the
+  // (level, index) keys from (level, no index): only spill indices move
+  hipError_t e = run_fill_gated(out, dem, bc, nx, ny, kModeFill, TileShape{2, 2}, stream,
+                                nullptr, nullptr, 0u);
+  if (e != hipSuccess) return e;
+  a.v.levels = true;
   // (level, index) keys live in one 64-bit array between visits; the
   // separate level and index outputs are written alongside
-  hipError_t e = hipMallocAsync((void**)&a.v.keys, (size_t)nx * ny * 8, stream);
+  e = hipMallocAsync((void**)&a.v.keys, (size_t)nx * ny * 8, stream);
   if (e != hipSuccess) return e;
   e = dispatch_fill<kModeSpill>(a, ts, stream, stats);
   hipError_t f = hipFreeAsync(a.v.keys, stream);

@@ -45,6 +45,10 @@ struct FillView {
   const float* src;
   const uint8_t* bc;
   long long nx, ny;
+  // kModeSpill: state already holds the exact fill levels F, so a cell
+  // starts at (F, no index) instead of (+INF, no index): a tighter upper
+  // bound of its key, which leaves only the index part to settle
+  bool levels;
 };
 
 // ---------------------------------------------------------------- level types
@@ -95,21 +99,48 @@ struct KeyLevel {
   }
 };
 
+// Float levels as their order-preserving uint32 images (ord_bits): the
+// relaxation only compares, and integer min / max / min3 need none of the
+// NaN quieting that IEEE-mode float min / max cost. Fill levels are never
+// NaN (no-data cells are +INF), so the order is the float order.
+struct OrdLevel {
+  using T = unsigned;
+  static __device__ __forceinline__ T lo(T a, T b) { return a < b ? a : b; }
+  static __device__ __forceinline__ T hi(T a, T b) { return a < b ? b : a; }
+  static __device__ __forceinline__ bool less(T a, T b) { return a < b; }
+  static __device__ __forceinline__ T from_below(T v, T e) { return dpp_u32(e, v, 0x138); }
+  static __device__ __forceinline__ T from_above(T v, T e) { return dpp_u32(e, v, 0x130); }
+  static constexpr T kUnreached = 0xff800000u;  // ord_bits(+INF)
+  static constexpr T kInert = 0xff800000u;
+};
+
 template <int MODE>
-using FillLevel = typename std::conditional<MODE == kModeSpill, KeyLevel, FloatLevel>::type;
+using FillLevel = typename std::conditional<MODE == kModeSpill, KeyLevel, OrdLevel>::type;
 
 // One directional sweep over a 64x64 quadrant with lanes along columns
 // (vertical sweep: down = +y) or along rows (horizontal: right = +x).
-// The value carried between steps is the lane's previous cell; its two
-// side neighbours come from the adjacent lanes by DPP, with the quadrant's
-// outside column/row read from LDS for the edge lanes. LDS operands are
-// fetched two steps ahead; none of them is written in between (a lane's
-// cell is written only at its own step, the outside line belongs to another
-// wavefront or the halo), and the carried value is stored every step.
+// The sweep runs as NCH independent chains of 64 / NCH steps, interleaved
+// step by step so the dependent min / max / DPP sequence of one chain
+// overlaps the others' (a lone wavefront per SIMD cannot hide that
+// latency otherwise). Each chain starts from the value its predecessor's
+// last cell held before the sweep; the relaxation reaches the same fixed
+// point in any order, information only crosses a chain boundary one
+// sweep later. The value carried between steps is the lane's previous
+// cell; its two side neighbours come from the adjacent lanes by DPP, with
+// the quadrant's outside column/row read from LDS for the edge lanes. LDS
+// operands are fetched kPrefetch steps ahead (the loop is fully unrolled,
+// so the windows live in registers); none of them is written in between
+// (a lane's cell is written only at its own step, the outside line belongs
+// to another wavefront or the halo), and the carried value is stored every
+// step.
+constexpr int kPrefetch = 4;
+constexpr int kChains = 2;
+
 template <class G, class L, bool VERT, bool FWD>
 __device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::T* sd, int R0,
                                            int C0, int lane) {
   using T = typename L::T;
+  constexpr int NCH = kChains, LEN = 64 / kChains;
   constexpr int step = FWD ? 1 : -1;
   bool changed = false;
   const int own = VERT ? C0 + lane : R0 + lane;      // lane's column / row
@@ -117,27 +148,38 @@ __device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::
   const int k0 = VERT ? (FWD ? R0 : R0 + 63) : (FWD ? C0 : C0 + 63);
   auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
   auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
-  T pc = sf[cell(k0 - step)];
-  T f0 = sf[cell(k0)], d0 = sd[cell(k0)], e0 = sf[edge_cell(k0 - step)];
-  T f1 = sf[cell(k0 + step)], d1 = sd[cell(k0 + step)], e1 = sf[edge_cell(k0)];
-  int k = k0;
-#pragma unroll 4
-  for (int s = 0; s < 64; s++, k += step) {
-    const T fc = f0, dv = d0, edge = e0;
-    f0 = f1;
-    d0 = d1;
-    e0 = e1;
-    if (s < 62) {
-      f1 = sf[cell(k + 2 * step)];
-      d1 = sd[cell(k + 2 * step)];
-      e1 = sf[edge_cell(k + step)];
+  T pc[NCH];
+  T fw[NCH][kPrefetch], dw[NCH][kPrefetch], ew[NCH][kPrefetch];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int kc = k0 + c * LEN * step;
+    pc[c] = sf[cell(kc - step)];
+#pragma unroll
+    for (int i = 0; i < kPrefetch; i++) {
+      fw[c][i] = sf[cell(kc + i * step)];
+      dw[c][i] = sd[cell(kc + i * step)];
+      ew[c][i] = sf[edge_cell(kc + (i - 1) * step)];
     }
-    const T a = L::from_below(pc, edge);
-    const T b = L::from_above(pc, edge);
-    const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc, fc)));
-    changed |= L::less(nv, fc);
-    pc = L::lo(nv, fc);
-    sf[cell(k)] = pc;
+  }
+#pragma unroll
+  for (int s = 0; s < LEN; s++) {
+    const int slot = s % kPrefetch;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int k = k0 + (c * LEN + s) * step;
+      const T fc = fw[c][slot], dv = dw[c][slot], edge = ew[c][slot];
+      if (s + kPrefetch < LEN) {
+        fw[c][slot] = sf[cell(k + kPrefetch * step)];
+        dw[c][slot] = sd[cell(k + kPrefetch * step)];
+        ew[c][slot] = sf[edge_cell(k + (kPrefetch - 1) * step)];
+      }
+      const T a = L::from_below(pc[c], edge);
+      const T b = L::from_above(pc[c], edge);
+      const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc[c], fc)));
+      changed |= L::less(nv, fc);
+      pc[c] = L::lo(nv, fc);
+      sf[cell(k)] = pc[c];
+    }
   }
   return changed;
 }
@@ -171,6 +213,96 @@ __device__ __forceinline__ bool fill_violation(const typename L::T* sf, const ty
     emid = edn;
   }
   return v;
+}
+
+// ---------------------------------------------------------------- register sweeps
+// Order-domain (uint32) sweeps with the wavefront's 64x64 quadrant held in
+// registers: f[i] is cell (R0 + i, C0 + lane) in the column layout
+// (vertical sweeps) and cell (R0 + lane, C0 + i) in the row layout
+// (horizontal sweeps); the layouts switch through the tile in LDS. Only
+// the bound d comes from LDS during a sweep (independent loads, hoisted
+// freely), so no step waits on LDS. Each sweep applies the neighbours
+// behind it (down: N, NW, NE; up: S, SW, SE; right: W, NW, SW; left: E,
+// NE, SE) with lanes' side values by DPP. Lanes 0 and 63 take the
+// quadrant's outside line only where the perpendicular sweeps cannot
+// supply it: the carried start value of each sweep is the outside row /
+// column, and the vertical sweeps' first step reads the four corner cells;
+// every other relation to outside cells is applied by a perpendicular
+// sweep, so together the four sweeps apply all 8 neighbour relations of
+// every cell (the fixed-point test then finds nothing a sweep cannot fix).
+// Each sweep runs as two interleaved chains of 32 steps (the second starts
+// from the old value at the split) so one chain's dependent DPP / min /
+// med3 sequence overlaps the other's.
+constexpr unsigned kNoNbr = 0xffffffffu;  // above every level: min() ignores it
+
+template <class G, bool VERT, bool FWD>
+__device__ __forceinline__ bool ord_sweep_regs(unsigned (&f)[64], const unsigned* sf,
+                                               const unsigned* sd, int R0, int C0, int lane) {
+  // start value: the outside row (vertical) or column (horizontal)
+  const int out_k = FWD ? -1 : 64;
+  unsigned pc0 = VERT ? sf[G::at(R0 + out_k, C0 + lane)] : sf[G::at(R0 + lane, C0 + out_k)];
+  // first-step DPP fill-in for lanes 0 / 63: the corner cells (vertical only)
+  unsigned corner = kNoNbr;
+  if (VERT) {
+    const unsigned lo = sf[G::at(R0 + out_k, C0 - 1)], hi = sf[G::at(R0 + out_k, C0 + 64)];
+    corner = (lane == 0) ? lo : hi;
+  }
+  auto dcell = [&](int i) {
+    return VERT ? sd[G::at(R0 + i, C0 + lane)] : sd[G::at(R0 + lane, C0 + i)];
+  };
+  unsigned changed = 0u;
+  unsigned pc1 = f[FWD ? 31 : 32];  // second chain starts from the old split value
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+    const int i0 = FWD ? s : 63 - s;         // chain 0
+    const int i1 = FWD ? 32 + s : 31 - s;    // chain 1
+    const unsigned d0 = dcell(i0), d1 = dcell(i1);
+    const unsigned old0 = (s == 0) ? corner : kNoNbr;
+    const unsigned a0 = dpp_u32(old0, pc0, 0x138), b0 = dpp_u32(old0, pc0, 0x130);
+    const unsigned a1 = dpp_u32(kNoNbr, pc1, 0x138), b1 = dpp_u32(kNoNbr, pc1, 0x130);
+    const unsigned m0 = min(min(a0, b0), pc0), m1 = min(min(a1, b1), pc1);
+    const unsigned f0 = f[i0], f1 = f[i1];
+    const unsigned n0 = max(d0, min(m0, f0)), n1 = max(d1, min(m1, f1));
+    changed |= (unsigned)(n0 < f0) | (unsigned)(n1 < f1);
+    f[i0] = n0;
+    f[i1] = n1;
+    pc0 = n0;
+    pc1 = n1;
+  }
+  return changed != 0u;
+}
+
+// Relaxes a tile to its local fixed point with the register sweeps; the
+// tile's state is in sf before and after. Returns the sweep iterations.
+template <class G>
+__device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int R0, int C0,
+                                          int lane, unsigned max_local, int* any_out,
+                                          int* changed_out) {
+  unsigned f[64];
+  unsigned it = 0;
+  int any = __syncthreads_or(fill_violation<G, OrdLevel>(sf, sd, R0, C0, lane) ? 1 : 0);
+  int any_change = 0;
+  while (any && it < max_local) {
+#pragma unroll
+    for (int r = 0; r < 64; r++) f[r] = sf[G::at(R0 + r, C0 + lane)];
+    bool ch = ord_sweep_regs<G, true, true>(f, sf, sd, R0, C0, lane);
+    ch |= ord_sweep_regs<G, true, false>(f, sf, sd, R0, C0, lane);
+#pragma unroll
+    for (int r = 0; r < 64; r++) sf[G::at(R0 + r, C0 + lane)] = f[r];
+#pragma unroll
+    for (int c = 0; c < 64; c++) f[c] = sf[G::at(R0 + lane, C0 + c)];
+    ch |= ord_sweep_regs<G, false, true>(f, sf, sd, R0, C0, lane);
+    ch |= ord_sweep_regs<G, false, false>(f, sf, sd, R0, C0, lane);
+#pragma unroll
+    for (int c = 0; c < 64; c++) sf[G::at(R0 + lane, C0 + c)] = f[c];
+    any = __syncthreads_or(ch ? 1 : 0);
+    any_change |= any;
+    it++;
+    if (any) any = __syncthreads_or(fill_violation<G, OrdLevel>(sf, sd, R0, C0, lane) ? 1 : 0);
+  }
+  *any_out = any;
+  *changed_out = any_change;
+  return it;
 }
 
 // LDS of one fill visit.
@@ -219,7 +351,10 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
           const float raw = v.src[g];
           if (raw == raw) {
             d = KeyLevel::make(raw, (unsigned)g);
-            f = first ? ((v.bc[g] == 1) ? d : KeyLevel::kUnreached) : v.keys[g];
+            f = first ? ((v.bc[g] == 1) ? d
+                                        : (v.levels ? KeyLevel::make(v.state[g], 0xffffffffu)
+                                                    : KeyLevel::kUnreached))
+                      : v.keys[g];
           }
         }
         fv[u] = f;
@@ -239,14 +374,17 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   } else {
     // raw inputs first (bound source in sd; bc flag or stored state in sf),
     // then one LDS pass turns them into (level, bound)
-    const StageF32 src{v.src, sd, __builtin_nanf("")};
+    // floats staged in place, then turned into ord_bits words
+    float* sdf = reinterpret_cast<float*>(sd);
+    float* sff = reinterpret_cast<float*>(sf);
+    const StageF32 src{v.src, sdf, __builtin_nanf("")};
     if (MODE == kModeFill && first)
-      stage_tile<G>(src, StageU8{v.bc, sf, 0.f}, true, x0, y0, v.nx, v.ny);
+      stage_tile<G>(src, StageU8{v.bc, sff, 0.f}, true, x0, y0, v.nx, v.ny);
     else
-      stage_tile<G>(src, StageF32{v.state, sf, kInf}, true, x0, y0, v.nx, v.ny);
+      stage_tile<G>(src, StageF32{v.state, sff, kInf}, true, x0, y0, v.nx, v.ny);
     __syncthreads();
     for (int k = tid; k < G::CELLS; k += G::THREADS) {
-      float d = sd[k], f = sf[k];
+      float d = sdf[k], f = sff[k];
       if (MODE == kModeFill) {
         if (first) f = (f == 1.f) ? d : kInf;
       } else {
@@ -257,8 +395,8 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
         d = kInf;
         f = kInf;
       }
-      sd[k] = d;
-      sf[k] = f;
+      sd[k] = ord_bits(d);
+      sf[k] = ord_bits(f);
     }
   }
   __syncthreads();
@@ -275,9 +413,10 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
         sf[G::at(r, c)] =
             from_state ? v.keys[g]
                        : ((v.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
-                                         : KeyLevel::kUnreached);
+                                         : (v.levels ? KeyLevel::make(v.state[g], 0xffffffffu)
+                                                     : KeyLevel::kUnreached));
       } else {
-        sf[G::at(r, c)] = from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf);
+        sf[G::at(r, c)] = ord_bits(from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf));
       }
     });
     __syncthreads();
@@ -287,17 +426,21 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // ---- relax: sweep iterations until the parallel fixed-point test
   // passes (also tried before the first one)
   unsigned it = 0;
-  int any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
-  int any_change = 0;
-  while (any && it < max_local) {
-    bool ch = fill_sweep<G, L, true, true>(sf, sd, R0, C0, lane);
-    ch |= fill_sweep<G, L, true, false>(sf, sd, R0, C0, lane);
-    ch |= fill_sweep<G, L, false, true>(sf, sd, R0, C0, lane);
-    ch |= fill_sweep<G, L, false, false>(sf, sd, R0, C0, lane);
-    any = __syncthreads_or(ch ? 1 : 0);
-    any_change |= any;
-    it++;
-    if (any) any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+  int any = 0, any_change = 0;
+  if constexpr (MODE == kModeSpill) {
+    any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+    while (any && it < max_local) {
+      bool ch = fill_sweep<G, L, true, true>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, true, false>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, false, true>(sf, sd, R0, C0, lane);
+      ch |= fill_sweep<G, L, false, false>(sf, sd, R0, C0, lane);
+      any = __syncthreads_or(ch ? 1 : 0);
+      any_change |= any;
+      it++;
+      if (any) any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+    }
+  } else {
+    it = ord_relax_regs<G>(sf, sd, R0, C0, lane, max_local, &any, &any_change);
   }
   const unsigned long long c2 = wall_clock64();
   // ---- write back: the first visit initialises the tile's state; later
@@ -318,7 +461,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
           v.state[g] = inert ? v.src[g] : unord_bits((unsigned)(val >> 32));
           v.spill[g] = inert ? -1 : (int)(unsigned)val;
         } else {
-          float out = val;
+          float out = unord_bits(val);
           if (inert) {
             const float raw = v.src[g];
             if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
