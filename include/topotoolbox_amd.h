@@ -3,8 +3,10 @@
   excess-topography library.
 
   Part 1 declares the reference's own entry points with the reference's
-  exact signatures, so a C caller of libtopotoolbox links against
-  libtopotoolbox_amd.so unchanged. They take host pointers, copy to the
+  exact signatures (has_topotoolbox, fillsinks, reconstruct,
+  excesstopography_fsm2d, and their hybrid / fast-marching siblings), so a
+  C caller of libtopotoolbox links against libtopotoolbox_amd.so
+  unchanged. They take host pointers, copy to the
   GPU, run the HIP kernels and copy back before returning.
 
   Part 2 adds device-pointer entry points (stream-ordered, for ctypes /
@@ -57,6 +59,28 @@ void reconstruct(float *marker, float *mask, ptrdiff_t dims[2]);
    include/topotoolbox.h:607-609 (src/excesstopography.c:47-111). */
 void excesstopography_fsm2d(float *excess, float *dem, float *threshold_slopes,
                             float cellsize, ptrdiff_t dims[2]);
+
+/* Replaces fillsinks_hybrid, reference include/topotoolbox.h:351-352
+   (src/fillsinks.c:50-77). Same output as fillsinks (the reference's hybrid
+   algorithm reaches the same reconstruction); `queue` is scratch the
+   reference needs for its FIFO and is not used here. */
+void fillsinks_hybrid(float *output, ptrdiff_t *queue, float *dem, uint8_t *bc,
+                      ptrdiff_t dims[2]);
+
+/* Replaces reconstruct_hybrid, reference include/topotoolbox.h:1069-1071
+   (src/reconstruct.c:291-312). Same output as reconstruct; `queue` is not
+   used. Returns 0 (converged), or 1 if the call failed (the reference
+   returns 1 when its queue overflowed twice without converging). */
+int reconstruct_hybrid(float *marker, ptrdiff_t *queue, float *mask, ptrdiff_t dims[2]);
+
+/* Replaces excesstopography_fmm2d, reference include/topotoolbox.h:694-697
+   (src/excesstopography.c:113-172). Served by the GPU fast sweeping kernel:
+   both methods solve the same constrained eikonal equation, and the
+   reference's own tests require them to agree within 1e-4
+   (test/excesstopography.cpp:69-82,133); `heap` and `back` (priority-queue
+   scratch of the reference) are not used. */
+void excesstopography_fmm2d(float *excess, ptrdiff_t *heap, ptrdiff_t *back, float *dem,
+                            float *threshold_slopes, float cellsize, ptrdiff_t dims[2]);
 
 /* ---------------------------------------------------------------------
    Part 2: device-pointer API (no reference counterpart)

@@ -8,23 +8,29 @@ and the batched tile scheduler (batched).
 """
 from . import _lib
 from .ops import (
+    excesstopography_fmm2d_host,
     excesstopography_fsm2d,
     excesstopography_fsm2d_host,
     fillsinks,
     fillsinks_host,
+    fillsinks_hybrid_host,
     reconstruct,
     reconstruct_host,
+    reconstruct_hybrid_host,
     set_fill_method,
     set_tile_shape,
 )
 
 __all__ = [
+    "excesstopography_fmm2d_host",
     "excesstopography_fsm2d",
     "excesstopography_fsm2d_host",
     "fillsinks",
     "fillsinks_host",
+    "fillsinks_hybrid_host",
     "reconstruct",
     "reconstruct_host",
+    "reconstruct_hybrid_host",
     "set_fill_method",
     "set_tile_shape",
 ]

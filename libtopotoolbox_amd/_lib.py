@@ -82,6 +82,9 @@ SIGNATURES = {
     "fillsinks": (None, [_vp, _vp, _vp, _dims_t]),
     "reconstruct": (None, [_vp, _vp, _dims_t]),
     "excesstopography_fsm2d": (None, [_vp, _vp, _vp, ctypes.c_float, _dims_t]),
+    "fillsinks_hybrid": (None, [_vp, _vp, _vp, _vp, _dims_t]),
+    "reconstruct_hybrid": (ctypes.c_int, [_vp, _vp, _vp, _dims_t]),
+    "excesstopography_fmm2d": (None, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _dims_t]),
     "tt_fillsinks_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_reconstruct_dev": (ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_excesstopography_fsm2d_dev": (

@@ -320,4 +320,24 @@ void excesstopography_fsm2d(float* excess, float* dem, float* threshold_slopes, 
   }
 }
 
+void fillsinks_hybrid(float* output, ptrdiff_t* queue, float* dem, uint8_t* bc,
+                      ptrdiff_t dims[2]) {
+  (void)queue;
+  fillsinks(output, dem, bc, dims);
+}
+
+int reconstruct_hybrid(float* marker, ptrdiff_t* queue, float* mask, ptrdiff_t dims[2]) {
+  (void)queue;
+  tt::set_error("%s", "");
+  reconstruct(marker, mask, dims);
+  return tt::last_error()[0] ? 1 : 0;
+}
+
+void excesstopography_fmm2d(float* excess, ptrdiff_t* heap, ptrdiff_t* back, float* dem,
+                            float* threshold_slopes, float cellsize, ptrdiff_t dims[2]) {
+  (void)heap;
+  (void)back;
+  excesstopography_fsm2d(excess, dem, threshold_slopes, cellsize, dims);
+}
+
 }  // extern "C"

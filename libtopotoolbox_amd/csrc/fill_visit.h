@@ -421,6 +421,30 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
     });
     __syncthreads();
   }
+  // spill keys from known levels, first visit: an unfilled cell (level ==
+  // its own d) with a strictly lower neighbour keeps (d, own index) at the
+  // fixed point; seeding it there leaves only plateau cells to settle
+  if constexpr (MODE == kModeSpill) {
+    if (first && v.levels) {
+      for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
+        const int r = idx / G::TX, c = idx % G::TX;
+        const T k = sf[G::at(r, c)], d = sd[G::at(r, c)];
+        if ((unsigned)k != 0xffffffffu || d == KeyLevel::kInert) continue;
+        const unsigned lev = (unsigned)(k >> 32);
+        if (lev != (unsigned)(d >> 32)) continue;  // filled: level above d
+        unsigned lowest = 0xffffffffu;
+#pragma unroll
+        for (int dr = -1; dr <= 1; dr++)
+#pragma unroll
+          for (int dc = -1; dc <= 1; dc++) {
+            const unsigned l = (unsigned)(sf[G::at(r + dr, c + dc)] >> 32);
+            lowest = l < lowest ? l : lowest;
+          }
+        if (lowest < lev) sf[G::at(r, c)] = d;
+      }
+      __syncthreads();
+    }
+  }
   save_perimeter<G>(lds.sper, sf);
   const unsigned long long c1 = wall_clock64();
   // ---- relax: sweep iterations until the parallel fixed-point test

@@ -229,3 +229,54 @@ def excesstopography_fsm2d_host(dem: np.ndarray, slopes: np.ndarray,
     if err:
         raise _lib.TTError(err)
     return out
+
+
+def fillsinks_hybrid_host(dem: np.ndarray, bc: np.ndarray) -> np.ndarray:
+    """Reference-signature fillsinks_hybrid (the queue argument is scratch
+    the GPU path does not need; same output as fillsinks)."""
+    dem = _host(dem, np.float32)
+    bc = _host(bc, np.uint8)
+    out = np.empty_like(dem)
+    queue = np.empty(dem.shape, dtype=np.intp)
+    ny, nx = dem.shape
+    _lib.lib().fillsinks_hybrid(out.ctypes.data, queue.ctypes.data, dem.ctypes.data,
+                                bc.ctypes.data, _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out
+
+
+def reconstruct_hybrid_host(marker: np.ndarray, mask: np.ndarray):
+    """Reference-signature reconstruct_hybrid: (marker after reconstruction,
+    return code: 0 when converged)."""
+    m = _host(marker, np.float32).copy()
+    mask = _host(mask, np.float32)
+    queue = np.empty(m.shape, dtype=np.intp)
+    ny, nx = m.shape
+    rc = _lib.lib().reconstruct_hybrid(m.ctypes.data, queue.ctypes.data, mask.ctypes.data,
+                                       _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return m, rc
+
+
+def excesstopography_fmm2d_host(dem: np.ndarray, slopes: np.ndarray,
+                                cellsize: float) -> np.ndarray:
+    """Reference-signature excesstopography_fmm2d (heap / back are scratch of
+    the reference's priority queue; the GPU fast sweeping kernel solves the
+    same equation)."""
+    dem = _host(dem, np.float32)
+    slopes = _host(slopes, np.float32)
+    out = np.empty_like(dem)
+    heap = np.empty(dem.shape, dtype=np.intp)
+    back = np.empty(dem.shape, dtype=np.intp)
+    ny, nx = dem.shape
+    _lib.lib().excesstopography_fmm2d(out.ctypes.data, heap.ctypes.data, back.ctypes.data,
+                                      dem.ctypes.data, slopes.ctypes.data, float(cellsize),
+                                      _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out

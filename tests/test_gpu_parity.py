@@ -170,6 +170,33 @@ def test_reconstruct(cuda, oracle_lib):
     assert_bitwise(out.cpu().numpy(), want, "reconstruct random")
 
 
+def test_hybrid_and_fmm_dropins(cuda, oracle_lib):
+    """fillsinks_hybrid / reconstruct_hybrid (reference src/fillsinks.c:50-77,
+    src/reconstruct.c:291-312) give the same bits as the oracle's fillsinks /
+    reconstruct; excesstopography_fmm2d is within the reference's own
+    FSM-vs-FMM bound (1e-4, test/excesstopography.cpp:69-82) of the
+    fast-marching oracle, relative with denominator max(|u|, 1 m)."""
+    import libtopotoolbox_amd as tt
+
+    g = load_golden("fillsinks_nan_convention")
+    assert_bitwise(tt.fillsinks_hybrid_host(g["dem"], g["bc"]), g["out"], "fillsinks_hybrid")
+    rng = np.random.default_rng(21)
+    mask = oracle_lib.grid_random(79, (120, 170))
+    marker = np.where(rng.uniform(size=mask.shape) < 0.1, mask - 3.0, -np.inf).astype(np.float32)
+    want, _, conv = oracle_lib.reconstruct(marker, mask)
+    assert conv
+    got, rc = tt.reconstruct_hybrid_host(marker, mask)
+    assert rc == 0
+    assert_bitwise(got, want, "reconstruct_hybrid")
+    dem = (rng.uniform(size=(150, 130)) * 300).astype(np.float32)
+    slopes = rng.uniform(0.1, 0.9, size=dem.shape).astype(np.float32)
+    u = tt.excesstopography_fmm2d_host(dem, slopes, 30.0)
+    fsm_close(u, oracle_lib.fmm2d_f64(dem, slopes, 30.0), "fmm2d drop-in vs FP64 fast marching")
+    if oracle_lib.ref() is not None:
+        fsm_close(u, oracle_lib.ref_fmm2d(dem, slopes, 30.0).astype(np.float64),
+                  "fmm2d drop-in vs the compiled reference")
+
+
 def test_host_dropins(cuda, oracle_lib):
     import libtopotoolbox_amd as tt
 
