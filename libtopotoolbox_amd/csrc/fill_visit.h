@@ -421,11 +421,16 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
     });
     __syncthreads();
   }
+  // the perimeter as the neighbours last saw it (staged state), before
+  // anything below lowers it: every lowering from here on is compared with
+  // it to decide which neighbours to queue
+  save_perimeter<G>(lds.sper, sf);
   // spill keys from known levels, first visit: an unfilled cell (level ==
   // its own d) with a strictly lower neighbour keeps (d, own index) at the
   // fixed point; seeding it there leaves only plateau cells to settle
   if constexpr (MODE == kModeSpill) {
     if (first && v.levels) {
+      __syncthreads();  // perimeter saved before any cell is seeded
       for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
         const int r = idx / G::TX, c = idx % G::TX;
         const T k = sf[G::at(r, c)], d = sd[G::at(r, c)];
@@ -445,7 +450,6 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       __syncthreads();
     }
   }
-  save_perimeter<G>(lds.sper, sf);
   const unsigned long long c1 = wall_clock64();
   // ---- relax: sweep iterations until the parallel fixed-point test
   // passes (also tried before the first one)

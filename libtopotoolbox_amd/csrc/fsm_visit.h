@@ -36,54 +36,6 @@ struct FsmView {
   long long nx, ny;
 };
 
-// One Gauss-Seidel sweep over a 64x64 quadrant. VERT: lanes are columns
-// and the sweep walks rows (+y when FWD); the carried value is the updated
-// neighbour behind the sweep, the one ahead ("far") is the old value.
-// Otherwise lanes are rows and the sweep walks columns. LDS operands are
-// fetched two steps ahead (none of them is written in between) and the
-// lane's cell is stored every step.
-template <class G, bool VERT, bool FWD>
-__device__ __forceinline__ bool fsm_sweep(float* su, const float* sfi, int R0, int C0, int lane) {
-  constexpr int step = FWD ? 1 : -1;
-  bool changed = false;
-  const int own = VERT ? C0 + lane : R0 + lane;
-  const int side = (lane == 0) ? own - 1 : own + 1;
-  const int k0 = VERT ? (FWD ? R0 : R0 + 63) : (FWD ? C0 : C0 + 63);
-  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
-  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
-  float pc = su[cell(k0 - step)];
-  float c0 = su[cell(k0)], r0 = su[cell(k0 + step)], e0 = su[edge_cell(k0)], i0 = sfi[cell(k0)];
-  float c1 = r0, r1 = su[cell(k0 + 2 * step)], e1 = su[edge_cell(k0 + step)],
-        i1 = sfi[cell(k0 + step)];
-  int k = k0;
-#pragma unroll 2
-  for (int s = 0; s < 64; s++, k += step) {
-    const float cur = c0, far = r0, edge = e0, fi = i0;
-    c0 = c1;
-    r0 = r1;
-    e0 = e1;
-    i0 = i1;
-    if (s < 62) {
-      c1 = r1;  // the cell two steps ahead is read once, as "far" then "cur"
-      r1 = su[cell(k + 3 * step)];
-      e1 = su[edge_cell(k + 2 * step)];
-      i1 = sfi[cell(k + 2 * step)];
-    }
-    // across-lane neighbours (columns C-1 / C+1, or rows R-1 / R+1)
-    const float lo = lane_from_below(cur, edge);
-    const float hi = lane_from_above(cur, edge);
-    // u1 = fminf(x+1, x-1), u2 = fminf(y+1, y-1) as in the reference
-    const float along = FWD ? fminf(far, pc) : fminf(pc, far);
-    const float across = fminf(hi, lo);
-    const float prop = VERT ? godunov(across, along, fi) : godunov(along, across, fi);
-    const bool upd = prop < cur;
-    changed |= upd;
-    pc = upd ? prop : cur;
-    su[cell(k)] = pc;
-  }
-  return changed;
-}
-
 // Fixed-point test of a wavefront's quadrant: true (in some lane) if a cell
 // would accept its Godunov proposal. Lanes are columns walking the rows
 // (rows above / below in registers, side columns by DPP); the steps are
@@ -107,6 +59,99 @@ __device__ __forceinline__ bool fsm_violation(const float* su, const float* sfi,
     mid = dn;
   }
   return v;
+}
+
+// ---------------------------------------------------------------- register sweeps
+// The wavefront's 64x64 quadrant of u in registers: u[i] is cell
+// (R0 + i, C0 + lane) in the column layout (vertical sweeps) and cell
+// (R0 + lane, C0 + i) in the row layout (horizontal sweeps), switched
+// through the tile in LDS; fi comes from LDS (independent loads). Along
+// the sweep the carried value is the updated cell behind and the register
+// ahead the old one (the outside row / column at the ends); across it the
+// lanes' old values of the same row / column by DPP. Lanes 0 and 63 take
+// the quadrant's outside line across the sweep only at a vertical sweep's
+// first and last step (the quadrant's corner cells); every other edge cell
+// has its full 4-neighbour proposal evaluated by the perpendicular sweeps,
+// and the corner cells by these steps, so every cell's full Godunov
+// proposal is applied by some sweep. Two interleaved chains of 32 steps
+// (the second starts from the old value at the split) overlap each
+// other's dependent update.
+template <class G, bool VERT, bool FWD>
+__device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su,
+                                               const float* sfi, int R0, int C0, int lane) {
+  constexpr int step = FWD ? 1 : -1;
+  const int out_k = FWD ? -1 : 64, end_k = FWD ? 64 : -1;
+  float pc[2];
+  pc[0] = VERT ? su[G::at(R0 + out_k, C0 + lane)] : su[G::at(R0 + lane, C0 + out_k)];
+  pc[1] = u[FWD ? 31 : 32];
+  const float beyond = VERT ? su[G::at(R0 + end_k, C0 + lane)] : su[G::at(R0 + lane, C0 + end_k)];
+  float e_first = kInf, e_last = kInf;
+  if (VERT) {
+    const int rf = FWD ? 0 : 63, rl = FWD ? 63 : 0;
+    e_first = su[G::at(R0 + rf, lane == 0 ? C0 - 1 : C0 + 64)];
+    e_last = su[G::at(R0 + rl, lane == 0 ? C0 - 1 : C0 + 64)];
+  }
+  auto fcell = [&](int i) {
+    return VERT ? sfi[G::at(R0 + i, C0 + lane)] : sfi[G::at(R0 + lane, C0 + i)];
+  };
+  bool changed = false;
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int i = c == 0 ? (FWD ? s : 63 - s) : (FWD ? 32 + s : 31 - s);
+      const int gs = c * 32 + s;  // step of the whole sweep
+      const float cur = u[i];
+      const int j = i + step;
+      const float far = (j >= 0 && j < 64) ? u[j] : beyond;
+      const float old = (VERT && gs == 0) ? e_first : ((VERT && gs == 63) ? e_last : kInf);
+      const float lo = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(old), __float_as_int(cur), 0x138, 0xf, 0xf, false));
+      const float hi = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(old), __float_as_int(cur), 0x130, 0xf, 0xf, false));
+      const float across = fminf(hi, lo);
+      const float along = FWD ? fminf(far, pc[c]) : fminf(pc[c], far);
+      const float prop = VERT ? godunov(across, along, fcell(i)) : godunov(along, across, fcell(i));
+      const bool upd = prop < cur;
+      changed |= upd;
+      const float nv = upd ? prop : cur;
+      u[i] = nv;
+      pc[c] = nv;
+    }
+  }
+  return changed;
+}
+
+// Relaxes the tile to its local fixed point with the register sweeps (state
+// in su before and after); returns the sweep iterations.
+template <class G>
+__device__ inline unsigned fsm_relax_regs(float* su, const float* sfi, int R0, int C0, int lane,
+                                          unsigned max_local, int* any_out, int* changed_out) {
+  float u[64];
+  unsigned it = 0;
+  int any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
+  int any_change = 0;
+  while (any && it < max_local) {
+#pragma unroll
+    for (int r = 0; r < 64; r++) u[r] = su[G::at(R0 + r, C0 + lane)];
+    bool ch = fsm_sweep_regs<G, true, true>(u, su, sfi, R0, C0, lane);
+    ch |= fsm_sweep_regs<G, true, false>(u, su, sfi, R0, C0, lane);
+#pragma unroll
+    for (int r = 0; r < 64; r++) su[G::at(R0 + r, C0 + lane)] = u[r];
+#pragma unroll
+    for (int c = 0; c < 64; c++) u[c] = su[G::at(R0 + lane, C0 + c)];
+    ch |= fsm_sweep_regs<G, false, true>(u, su, sfi, R0, C0, lane);
+    ch |= fsm_sweep_regs<G, false, false>(u, su, sfi, R0, C0, lane);
+#pragma unroll
+    for (int c = 0; c < 64; c++) su[G::at(R0 + lane, C0 + c)] = u[c];
+    any = __syncthreads_or(ch ? 1 : 0);
+    any_change |= any;
+    it++;
+    if (any) any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
+  }
+  *any_out = any;
+  *changed_out = any_change;
+  return it;
 }
 
 // LDS of one excess-topography visit.
@@ -149,19 +194,8 @@ __device__ inline unsigned fsm_visit(const FsmView& v, long long x0, long long y
   __syncthreads();
   save_perimeter<G>(lds.sper, su_bits);
   const unsigned long long c1 = wall_clock64();
-  unsigned it = 0;
-  int any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
-  int any_change = 0;
-  while (any && it < max_local) {
-    bool ch = fsm_sweep<G, true, true>(su, sfi, R0, C0, lane);
-    ch |= fsm_sweep<G, true, false>(su, sfi, R0, C0, lane);
-    ch |= fsm_sweep<G, false, true>(su, sfi, R0, C0, lane);
-    ch |= fsm_sweep<G, false, false>(su, sfi, R0, C0, lane);
-    any = __syncthreads_or(ch ? 1 : 0);
-    any_change |= any;
-    it++;
-    if (any) any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
-  }
+  int any = 0, any_change = 0;
+  const unsigned it = fsm_relax_regs<G>(su, sfi, R0, C0, lane, max_local, &any, &any_change);
   const unsigned long long c2 = wall_clock64();
   // the first visit initialises the output; later visits write changed
   // tiles only

@@ -74,6 +74,29 @@ def test_spill_index_matches_oracle(cuda, oracle_lib, tile):
         tt.set_tile_shape(2, 2)
 
 
+def test_spill_index_seeded_tile_boundaries(cuda, oracle_lib):
+    """Regression: a 1681 x 936 fractal with 5% pits (seed 5096) on 128 x 64
+    tiles; spill cells seeded on a tile's first visit must still queue the
+    neighbours that staged the tile's unseeded values (three cells on tile
+    boundary rows differed from the oracle when they did not)."""
+    import libtopotoolbox_amd as tt
+    from libtopotoolbox_amd import ops, synthetic as S
+
+    dem = S.dem_pits(1681, 936, 5096)
+    bc = edge_bc(dem.shape)
+    want, wspill = oracle_lib.fillsinks_spill(dem, bc)
+    try:
+        for tile in [(2, 1), (1, 2), (1, 1)]:
+            tt.set_tile_shape(*tile)
+            (out, spill), st = ops.fillsinks_spill(to_gpu(dem), to_gpu(bc), stats=True)
+            assert st["error"] == 0
+            np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
+            np.testing.assert_array_equal(spill.cpu().numpy().astype(np.int64), wspill,
+                                          err_msg=f"tile {tile}")
+    finally:
+        tt.set_tile_shape(2, 2)
+
+
 def test_fill_backward_vs_fp64(cuda, oracle_lib):
     from libtopotoolbox_amd import ops
 
