@@ -82,6 +82,28 @@ int reconstruct_hybrid(float *marker, ptrdiff_t *queue, float *mask, ptrdiff_t d
 void excesstopography_fmm2d(float *excess, ptrdiff_t *heap, ptrdiff_t *back, float *dem,
                             float *threshold_slopes, float cellsize, ptrdiff_t dims[2]);
 
+/* Replaces identifyflats, reference include/topotoolbox.h:405
+   (src/identifyflats.c:32-121). output[p] & 1: flat, & 2: sill, & 4:
+   presill; returns the number of flats. */
+ptrdiff_t identifyflats(int32_t *output, float *dem, ptrdiff_t dims[2]);
+
+/* Replace min_filter and max_filter, reference include/topotoolbox.h:101-102
+   and :155-156 (src/value_filters.c:9-59, 61-110). The structuring element
+   is se_dims[0] x se_dims[1] x se_dims[2] bytes (slices folded in turn);
+   the result has the reference's bits, including the sign of zeros. */
+void min_filter(float *output, float *dem, uint8_t *structuring_element, ptrdiff_t io_dims[2],
+                ptrdiff_t se_dims[3]);
+void max_filter(float *output, float *dem, uint8_t *structuring_element, ptrdiff_t io_dims[2],
+                ptrdiff_t se_dims[3]);
+
+/* Replace min_filter_square and max_filter_square, reference
+   include/topotoolbox.h:204-205 and :253-254 (src/value_filters.c:112-178,
+   180-246). `tmp` receives the pass along dims[0], as in the reference. */
+void min_filter_square(float *output, float *dem, float *tmp, uint8_t width,
+                       ptrdiff_t io_dims[2]);
+void max_filter_square(float *output, float *dem, float *tmp, uint8_t width,
+                       ptrdiff_t io_dims[2]);
+
 /* ---------------------------------------------------------------------
    Part 2: device-pointer API (no reference counterpart)
    All pointers are device pointers; `stream` is a hipStream_t (NULL = the
@@ -141,6 +163,23 @@ int tt_excesstopography_fsm2d_backward_dev(float *g_dem, float *g_slopes, const 
                                            const float *dem, const float *slopes, float cellsize,
                                            const float *g_out, ptrdiff_t nx, ptrdiff_t ny,
                                            void *stream, tt_stats *stats);
+
+/* identifyflats on device buffers. With `count` non-NULL the call waits and
+   stores the number of flats there; with NULL it stays asynchronous. */
+int tt_identifyflats_dev(int32_t *output, const float *dem, ptrdiff_t nx, ptrdiff_t ny,
+                         void *stream, int64_t *count);
+
+/* min_filter (op < 0) or max_filter (op > 0) on device buffers;
+   `structuring_element` is device memory of se_dims[0]*se_dims[1]*se_dims[2]
+   bytes. */
+int tt_value_filter_dev(int op, float *output, const float *dem,
+                        const uint8_t *structuring_element, const ptrdiff_t se_dims[3],
+                        ptrdiff_t nx, ptrdiff_t ny, void *stream);
+
+/* min_filter_square (op < 0) or max_filter_square (op > 0) on device
+   buffers; `tmp` is device memory of nx * ny floats. */
+int tt_value_filter_square_dev(int op, float *output, const float *dem, float *tmp, int width,
+                               ptrdiff_t nx, ptrdiff_t ny, void *stream);
 
 /* ---------------------------------------------------------------------
    Batched entry (no reference counterpart: the reference handles one grid

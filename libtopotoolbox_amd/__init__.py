@@ -1,7 +1,7 @@
 """libtopotoolbox_amd: MI355X-native sink filling and excess topography.
 
 Drop-in for libtopotoolbox's fillsinks / reconstruct / excesstopography_fsm2d
-hot path. The compute runs in HIP kernels for gfx950 behind a C ABI
+hot path and the stencils next to it (identifyflats, min/max filters). The compute runs in HIP kernels for gfx950 behind a C ABI
 (include/topotoolbox_amd.h); this package is the host-side mirror of that
 interface for torch tensors (ops), the differentiable wrappers (autograd)
 and the batched tile scheduler (batched).
@@ -14,6 +14,16 @@ from .ops import (
     fillsinks,
     fillsinks_host,
     fillsinks_hybrid_host,
+    identifyflats,
+    identifyflats_host,
+    max_filter,
+    max_filter_host,
+    max_filter_square,
+    max_filter_square_host,
+    min_filter,
+    min_filter_host,
+    min_filter_square,
+    min_filter_square_host,
     reconstruct,
     reconstruct_host,
     reconstruct_hybrid_host,
@@ -28,6 +38,16 @@ __all__ = [
     "fillsinks",
     "fillsinks_host",
     "fillsinks_hybrid_host",
+    "identifyflats",
+    "identifyflats_host",
+    "max_filter",
+    "max_filter_host",
+    "max_filter_square",
+    "max_filter_square_host",
+    "min_filter",
+    "min_filter_host",
+    "min_filter_square",
+    "min_filter_square_host",
     "reconstruct",
     "reconstruct_host",
     "reconstruct_hybrid_host",

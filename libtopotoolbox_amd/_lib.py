@@ -75,6 +75,7 @@ BATCH_FORWARD_ONLY = 1
 _vp = ctypes.c_void_p
 _pd = ctypes.c_ssize_t  # ptrdiff_t
 _dims_t = _pd * 2
+_se_dims_t = _pd * 3
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -85,6 +86,11 @@ SIGNATURES = {
     "fillsinks_hybrid": (None, [_vp, _vp, _vp, _vp, _dims_t]),
     "reconstruct_hybrid": (ctypes.c_int, [_vp, _vp, _vp, _dims_t]),
     "excesstopography_fmm2d": (None, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _dims_t]),
+    "identifyflats": (ctypes.c_ssize_t, [_vp, _vp, _dims_t]),
+    "min_filter": (None, [_vp, _vp, _vp, _dims_t, _se_dims_t]),
+    "max_filter": (None, [_vp, _vp, _vp, _dims_t, _se_dims_t]),
+    "min_filter_square": (None, [_vp, _vp, _vp, ctypes.c_uint8, _dims_t]),
+    "max_filter_square": (None, [_vp, _vp, _vp, ctypes.c_uint8, _dims_t]),
     "tt_fillsinks_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_reconstruct_dev": (ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_excesstopography_fsm2d_dev": (
@@ -95,6 +101,12 @@ SIGNATURES = {
     "tt_excesstopography_fsm2d_backward_dev": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_identifyflats_dev": (
+        ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(ctypes.c_int64)]),
+    "tt_value_filter_dev": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _se_dims_t, _pd, _pd,
+                                           _vp]),
+    "tt_value_filter_square_dev": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                                  _pd, _pd, _vp]),
     "tt_batch_plan": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t),
                                      ctypes.POINTER(ctypes.c_size_t)]),
@@ -148,3 +160,7 @@ def check(status: int, what: str) -> None:
 
 def dims(nx: int, ny: int):
     return _dims_t(nx, ny)
+
+
+def se_dims(sx: int, sy: int, slices: int):
+    return _se_dims_t(sx, sy, slices)

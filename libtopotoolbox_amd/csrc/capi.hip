@@ -217,6 +217,46 @@ int tt_excesstopography_fsm2d_backward_dev(float* g_dem, float* g_slopes, const 
   return 0;
 }
 
+int tt_identifyflats_dev(int32_t* output, const float* dem, ptrdiff_t nx, ptrdiff_t ny,
+                         void* stream, int64_t* count) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_identifyflats_dev: invalid dims");
+    return 1;
+  }
+  hipError_t e = tt::run_identifyflats(output, dem, nx, ny, (hipStream_t)stream, count);
+  if (e != hipSuccess) return fail("tt_identifyflats_dev", e);
+  return 0;
+}
+
+int tt_value_filter_dev(int op, float* output, const float* dem,
+                        const uint8_t* structuring_element, const ptrdiff_t se_dims[3],
+                        ptrdiff_t nx, ptrdiff_t ny, void* stream) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || !se_dims || se_dims[0] < 0 || se_dims[1] < 0 || se_dims[2] < 0 ||
+      op == 0) {
+    tt::set_error("tt_value_filter_dev: invalid dims or op");
+    return 1;
+  }
+  hipError_t e = tt::run_value_filter(op, output, dem, structuring_element, se_dims, nx, ny,
+                                      (hipStream_t)stream);
+  if (e != hipSuccess) return fail("tt_value_filter_dev", e);
+  return 0;
+}
+
+int tt_value_filter_square_dev(int op, float* output, const float* dem, float* tmp, int width,
+                               ptrdiff_t nx, ptrdiff_t ny, void* stream) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || width < 0 || op == 0) {
+    tt::set_error("tt_value_filter_square_dev: invalid dims, width or op");
+    return 1;
+  }
+  hipError_t e =
+      tt::run_value_filter_square(op, output, dem, tmp, width, nx, ny, (hipStream_t)stream);
+  if (e != hipSuccess) return fail("tt_value_filter_square_dev", e);
+  return 0;
+}
+
 int tt_batch_plan(int op, const tt_batch_item* items, int nitems, int qx, int qy, void* plan,
                   size_t* plan_bytes, size_t* workspace_bytes) {
   tt::set_error("%s", "");
@@ -338,6 +378,113 @@ void excesstopography_fmm2d(float* excess, ptrdiff_t* heap, ptrdiff_t* back, flo
   (void)heap;
   (void)back;
   excesstopography_fsm2d(excess, dem, threshold_slopes, cellsize, dims);
+}
+
+ptrdiff_t identifyflats(int32_t* output, float* dem, ptrdiff_t dims[2]) {
+  if (!dims_ok(dims, "identifyflats")) return report("identifyflats"), 0;
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return 0;
+  DevBuf d_out, d_dem;
+  hipError_t e = d_out.alloc(n * 4);
+  if (e == hipSuccess) e = d_dem.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail("identifyflats", e);
+    return report("identifyflats"), 0;
+  }
+  int64_t count = 0;
+  if (tt_identifyflats_dev((int32_t*)d_out.p, (const float*)d_dem.p, dims[0], dims[1], nullptr,
+                           &count) != 0)
+    return report("identifyflats"), 0;
+  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail("identifyflats", e);
+    report("identifyflats");
+  }
+  return (ptrdiff_t)count;
+}
+
+}  // extern "C"
+
+namespace {
+
+void value_filter_host(const char* fn, int op, float* output, const float* dem,
+                       const uint8_t* se, const ptrdiff_t* io_dims, const ptrdiff_t* se_dims) {
+  if (!dims_ok(io_dims, fn)) return report(fn);
+  if (!se_dims || se_dims[0] < 0 || se_dims[1] < 0 || se_dims[2] < 0) {
+    tt::set_error("%s: invalid se_dims", fn);
+    return report(fn);
+  }
+  const size_t n = (size_t)io_dims[0] * (size_t)io_dims[1];
+  const size_t ns = (size_t)se_dims[0] * (size_t)se_dims[1] * (size_t)se_dims[2];
+  if (n == 0) return;
+  DevBuf d_out, d_dem, d_se;
+  hipError_t e = d_out.alloc(n * 4);
+  if (e == hipSuccess) e = d_dem.alloc(n * 4);
+  if (e == hipSuccess) e = d_se.alloc(ns);
+  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && ns) e = hipMemcpy(d_se.p, se, ns, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail(fn, e);
+    return report(fn);
+  }
+  if (tt_value_filter_dev(op, (float*)d_out.p, (const float*)d_dem.p, (const uint8_t*)d_se.p,
+                          se_dims, io_dims[0], io_dims[1], nullptr) != 0)
+    return report(fn);
+  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail(fn, e);
+    report(fn);
+  }
+}
+
+void value_filter_square_host(const char* fn, int op, float* output, const float* dem,
+                              float* tmp, int width, const ptrdiff_t* io_dims) {
+  if (!dims_ok(io_dims, fn)) return report(fn);
+  const size_t n = (size_t)io_dims[0] * (size_t)io_dims[1];
+  if (n == 0) return;
+  DevBuf d_out, d_dem, d_tmp;
+  hipError_t e = d_out.alloc(n * 4);
+  if (e == hipSuccess) e = d_dem.alloc(n * 4);
+  if (e == hipSuccess) e = d_tmp.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail(fn, e);
+    return report(fn);
+  }
+  if (tt_value_filter_square_dev(op, (float*)d_out.p, (const float*)d_dem.p, (float*)d_tmp.p,
+                                 width, io_dims[0], io_dims[1], nullptr) != 0)
+    return report(fn);
+  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && tmp) e = hipMemcpy(tmp, d_tmp.p, n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail(fn, e);
+    report(fn);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void min_filter(float* output, float* dem, uint8_t* structuring_element, ptrdiff_t io_dims[2],
+                ptrdiff_t se_dims[3]) {
+  value_filter_host("min_filter", -1, output, dem, structuring_element, io_dims, se_dims);
+}
+
+void max_filter(float* output, float* dem, uint8_t* structuring_element, ptrdiff_t io_dims[2],
+                ptrdiff_t se_dims[3]) {
+  value_filter_host("max_filter", 1, output, dem, structuring_element, io_dims, se_dims);
+}
+
+void min_filter_square(float* output, float* dem, float* tmp, uint8_t width,
+                       ptrdiff_t io_dims[2]) {
+  value_filter_square_host("min_filter_square", -1, output, dem, tmp, width, io_dims);
+}
+
+void max_filter_square(float* output, float* dem, float* tmp, uint8_t width,
+                       ptrdiff_t io_dims[2]) {
+  value_filter_square_host("max_filter_square", 1, output, dem, tmp, width, io_dims);
 }
 
 }  // extern "C"

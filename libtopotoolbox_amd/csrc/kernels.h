@@ -115,6 +115,16 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
                      void* workspace, size_t workspace_bytes, unsigned max_local, int flags,
                      hipStream_t stream, RunStats* stats);
 
+// Stencils next to the hot path (stencil.hip). count_host, when non-null,
+// receives the number of flats after a synchronisation.
+hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t ny,
+                             hipStream_t stream, int64_t* count_host);
+hipError_t run_value_filter(int op, float* out, const float* dem, const uint8_t* se,
+                            const int64_t se_dims[3], int64_t nx, int64_t ny,
+                            hipStream_t stream);
+hipError_t run_value_filter_square(int op, float* out, const float* dem, float* tmp, int width,
+                                   int64_t nx, int64_t ny, hipStream_t stream);
+
 // Thread-local error message set by the entry points.
 void set_error(const char* fmt, ...);
 

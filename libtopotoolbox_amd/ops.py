@@ -15,6 +15,8 @@ fillsinks (:256-293), reconstruct (:1045-1056), excesstopography_fsm2d
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -184,6 +186,97 @@ def excesstopography_fsm2d_backward(excess: torch.Tensor, dem: torch.Tensor,
     return _ret((g_dem, g_s), st)
 
 
+# ------------------------------------------------------------------ stencils
+def identifyflats(dem: torch.Tensor, count: bool = True):
+    """Flat / sill / presill classification (reference identifyflats,
+    include/topotoolbox.h:363-405): int32 flags with bit 1 flat, 2 sill,
+    4 presill. Returns (flags, number of flats), or only flags with
+    ``count=False`` (then the call does not synchronise)."""
+    _check_grid(dem, "dem")
+    out = torch.empty(dem.shape, dtype=torch.int32, device=dem.device)
+    ny, nx = dem.shape
+    n = ctypes.c_int64(0)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_identifyflats_dev(out.data_ptr(), dem.data_ptr(), nx, ny,
+                                             _stream_handle(),
+                                             ctypes.byref(n) if count else None)
+    _lib.check(rc, "identifyflats")
+    return (out, int(n.value)) if count else out
+
+
+def _se_dims(se) -> tuple:
+    """se_dims (sx, sy, slices) of a structuring element shaped (sy, sx) or
+    (slices, sy, sx), so that se_dims[0] is the fastest dimension."""
+    if se.ndim == 2:
+        return (se.shape[1], se.shape[0], 1)
+    if se.ndim == 3:
+        return (se.shape[2], se.shape[1], se.shape[0])
+    raise ValueError("structuring element must be 2-D (sy, sx) or 3-D (slices, sy, sx)")
+
+
+def _value_filter(op: int, dem: torch.Tensor, se: torch.Tensor, out):
+    _check_grid(dem, "dem")
+    if not isinstance(se, torch.Tensor) or se.dtype != torch.uint8:
+        raise TypeError("structuring element must be a uint8 torch.Tensor")
+    se = se.to(dem.device).contiguous()
+    if out is None:
+        out = torch.empty_like(dem)
+    else:
+        _check_grid(out, "out")
+        _same_shape(dem, ("out", out))
+    ny, nx = dem.shape
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_value_filter_dev(op, out.data_ptr(), dem.data_ptr(), se.data_ptr(),
+                                            _lib.se_dims(*_se_dims(se)), nx, ny,
+                                            _stream_handle())
+    _lib.check(rc, "max_filter" if op > 0 else "min_filter")
+    return out
+
+
+def min_filter(dem: torch.Tensor, se: torch.Tensor, out: torch.Tensor | None = None):
+    """Minimum over a structuring element (reference min_filter,
+    include/topotoolbox.h:58-102); NaN cells stay NaN."""
+    return _value_filter(-1, dem, se, out)
+
+
+def max_filter(dem: torch.Tensor, se: torch.Tensor, out: torch.Tensor | None = None):
+    """Maximum over a structuring element (reference max_filter,
+    include/topotoolbox.h:104-156); NaN cells stay NaN."""
+    return _value_filter(1, dem, se, out)
+
+
+def _value_filter_square(op: int, dem: torch.Tensor, width: int, out, tmp):
+    _check_grid(dem, "dem")
+    if not 0 <= int(width) <= 255:
+        raise ValueError("width must be in 0..255 (uint8_t in the reference)")
+    out = torch.empty_like(dem) if out is None else out
+    tmp = torch.empty_like(dem) if tmp is None else tmp
+    _check_grid(out, "out")
+    _check_grid(tmp, "tmp")
+    _same_shape(dem, ("out", out), ("tmp", tmp))
+    ny, nx = dem.shape
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_value_filter_square_dev(op, out.data_ptr(), dem.data_ptr(),
+                                                   tmp.data_ptr(), int(width), nx, ny,
+                                                   _stream_handle())
+    _lib.check(rc, "max_filter_square" if op > 0 else "min_filter_square")
+    return out
+
+
+def min_filter_square(dem: torch.Tensor, width: int, out: torch.Tensor | None = None,
+                      tmp: torch.Tensor | None = None):
+    """Minimum over a width x width square (reference min_filter_square,
+    include/topotoolbox.h:158-205), separably through ``tmp``."""
+    return _value_filter_square(-1, dem, width, out, tmp)
+
+
+def max_filter_square(dem: torch.Tensor, width: int, out: torch.Tensor | None = None,
+                      tmp: torch.Tensor | None = None):
+    """Maximum over a width x width square (reference max_filter_square,
+    include/topotoolbox.h:207-254), separably through ``tmp``."""
+    return _value_filter_square(1, dem, width, out, tmp)
+
+
 # ------------------------------------------------------------------ host drop-ins
 def _host(a, dtype) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=dtype)
@@ -280,3 +373,63 @@ def excesstopography_fmm2d_host(dem: np.ndarray, slopes: np.ndarray,
     if err:
         raise _lib.TTError(err)
     return out
+
+
+def identifyflats_host(dem: np.ndarray):
+    """Reference-signature identifyflats: (int32 flags, number of flats)."""
+    dem = _host(dem, np.float32)
+    out = np.empty(dem.shape, dtype=np.int32)
+    ny, nx = dem.shape
+    n = _lib.lib().identifyflats(out.ctypes.data, dem.ctypes.data, _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out, int(n)
+
+
+def _value_filter_host(name: str, dem: np.ndarray, se: np.ndarray) -> np.ndarray:
+    dem = _host(dem, np.float32)
+    se = np.ascontiguousarray(se, dtype=np.uint8)
+    out = np.empty_like(dem)
+    ny, nx = dem.shape
+    getattr(_lib.lib(), name)(out.ctypes.data, dem.ctypes.data, se.ctypes.data,
+                              _lib.dims(nx, ny), _lib.se_dims(*_se_dims(se)))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out
+
+
+def min_filter_host(dem: np.ndarray, se: np.ndarray) -> np.ndarray:
+    """Reference-signature min_filter; ``se`` shaped (sy, sx) or
+    (slices, sy, sx)."""
+    return _value_filter_host("min_filter", dem, se)
+
+
+def max_filter_host(dem: np.ndarray, se: np.ndarray) -> np.ndarray:
+    """Reference-signature max_filter; ``se`` shaped (sy, sx) or
+    (slices, sy, sx)."""
+    return _value_filter_host("max_filter", dem, se)
+
+
+def _value_filter_square_host(name: str, dem: np.ndarray, width: int):
+    dem = _host(dem, np.float32)
+    out = np.empty_like(dem)
+    tmp = np.empty_like(dem)
+    ny, nx = dem.shape
+    getattr(_lib.lib(), name)(out.ctypes.data, dem.ctypes.data, tmp.ctypes.data, int(width),
+                              _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out, tmp
+
+
+def min_filter_square_host(dem: np.ndarray, width: int):
+    """Reference-signature min_filter_square: (output, tmp)."""
+    return _value_filter_square_host("min_filter_square", dem, width)
+
+
+def max_filter_square_host(dem: np.ndarray, width: int):
+    """Reference-signature max_filter_square: (output, tmp)."""
+    return _value_filter_square_host("max_filter_square", dem, width)

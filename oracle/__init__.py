@@ -29,6 +29,8 @@ _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _dims_t = ctypes.c_int64 * 2
+_se_dims_t = ctypes.c_int64 * 3
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
 def build(ref: bool = False) -> None:
@@ -68,6 +70,13 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_int64]
         L.oracle_fmm2d_f64.restype = ctypes.c_int
         L.oracle_fmm2d_f64.argtypes = [_f64p, _f64p, _f64p, ctypes.c_double, _dims_t]
+        L.oracle_identifyflats.restype = ctypes.c_ssize_t
+        L.oracle_identifyflats.argtypes = [_i32p, _f32p, _dims_t]
+        L.oracle_value_filter.restype = None
+        L.oracle_value_filter.argtypes = [_f32p, _f32p, _u8p, _dims_t, _se_dims_t, ctypes.c_int]
+        L.oracle_value_filter_square.restype = None
+        L.oracle_value_filter_square.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int, _dims_t,
+                                                 ctypes.c_int]
         L.oracle_fsm2d_adjoint_f64.restype = ctypes.c_int
         L.oracle_fsm2d_adjoint_f64.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p,
                                                ctypes.c_double, _f64p, _dims_t]
@@ -94,6 +103,14 @@ def ref():
         R.pcg4d.restype = ctypes.c_float
         R.pcg4d.argtypes = [ctypes.c_uint32] * 4
         R.has_topotoolbox.restype = ctypes.c_int
+        R.identifyflats.restype = ctypes.c_ssize_t
+        R.identifyflats.argtypes = [_i32p, _f32p, _dims_t]
+        for name in ("min_filter", "max_filter"):
+            getattr(R, name).restype = None
+            getattr(R, name).argtypes = [_f32p, _f32p, _u8p, _dims_t, _se_dims_t]
+        for name in ("min_filter_square", "max_filter_square"):
+            getattr(R, name).restype = None
+            getattr(R, name).argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint8, _dims_t]
         _ref = R
     return _ref
 
@@ -253,3 +270,62 @@ def ref_fmm2d(dem, slopes, cellsize):
     R.excesstopography_fmm2d(u, heap, back, dem, _c(slopes, np.float32),
                              np.float32(cellsize), _dims(dem))
     return u
+
+
+# ------------------------------------------------------------------ flats, value filters
+def _se_dims(se: np.ndarray):
+    """(ny, nx) or (slices, ny, nx) numpy SE -> se_dims = (nx, ny, slices)."""
+    se = np.asarray(se)
+    if se.ndim == 2:
+        se = se[None]
+    return np.ascontiguousarray(se, np.uint8), _se_dims_t(se.shape[2], se.shape[1], se.shape[0])
+
+
+def identifyflats(dem):
+    """(int32 bit field, number of flats): bit 0 flat, 1 sill, 2 presill."""
+    dem = _c(dem, np.float32)
+    out = np.empty(dem.shape, np.int32)
+    n = lib().oracle_identifyflats(out, dem, _dims(dem))
+    return out, int(n)
+
+
+def value_filter(dem, se, op: str):
+    """min_filter (op "min") / max_filter (op "max") over structuring element se."""
+    dem = _c(dem, np.float32)
+    se, sd = _se_dims(se)
+    out = np.empty_like(dem)
+    lib().oracle_value_filter(out, dem, se, _dims(dem), sd, -1 if op == "min" else 1)
+    return out
+
+
+def value_filter_square(dem, width: int, op: str):
+    dem = _c(dem, np.float32)
+    out = np.empty_like(dem)
+    tmp = np.empty_like(dem)
+    lib().oracle_value_filter_square(out, dem, tmp, int(width), _dims(dem),
+                                     -1 if op == "min" else 1)
+    return out
+
+
+def ref_identifyflats(dem):
+    dem = _c(dem, np.float32).copy()
+    out = np.empty(dem.shape, np.int32)
+    n = ref().identifyflats(out, dem, _dims(dem))
+    return out, int(n)
+
+
+def ref_value_filter(dem, se, op: str):
+    dem = _c(dem, np.float32).copy()
+    se, sd = _se_dims(se)
+    out = np.empty_like(dem)
+    getattr(ref(), f"{op}_filter")(out, dem, se, _dims(dem), sd)
+    return out
+
+
+def ref_value_filter_square(dem, width: int, op: str):
+    dem = _c(dem, np.float32).copy()
+    out = np.empty_like(dem)
+    tmp = np.empty_like(dem)
+    getattr(ref(), f"{op}_filter_square")(out, dem, tmp, int(width), _dims(dem))
+    return out
+

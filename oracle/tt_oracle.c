@@ -23,6 +23,11 @@
     oracle_fmm2d_f64             src/excesstopography.c:113-172 (fast
                                  marching), double precision cross-check
     oracle_fsm2d_adjoint_f64     adjoint of the converged Godunov system
+    oracle_identifyflats         src/identifyflats.c:32-121
+    oracle_value_filter          src/value_filters.c:9-113 (min_filter,
+                                 max_filter: op = -1 / +1)
+    oracle_value_filter_square   src/value_filters.c:115-246
+                                 (min_filter_square, max_filter_square)
 
   Pinning (which check fixes which function) is stated in the tests that
   exercise them: tests/test_oracle.py.
@@ -592,3 +597,126 @@ int oracle_fsm2d_adjoint_f64(double *g_dem, double *g_slopes, const double *u,
   free(lam);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Flats, sills, presills; value filters                              */
+/* ------------------------------------------------------------------ */
+
+/* Bit 0: flat (an interior cell equal to the minimum of its 3x3
+   neighbourhood, NaN neighbours counting as -INF); bit 1: sill (a non-flat
+   cell next to a flat of equal height); bit 2: presill (such a flat).
+   Returns the number of flats. src/identifyflats.c:32-121. */
+ptrdiff_t oracle_identifyflats(int32_t *out, const float *dem, const ptrdiff_t dims[2]) {
+  const ptrdiff_t nx = dims[0], ny = dims[1];
+  ptrdiff_t flats = 0;
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      const ptrdiff_t p = j * nx + i;
+      out[p] = 0;
+      if (i == 0 || j == 0 || i == nx - 1 || j == ny - 1) continue;
+      float lowest = dem[p];
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          if (!di && !dj) continue;
+          const float h = dem[p + dj * nx + di];
+          lowest = fminf(lowest, isnan(h) ? -INFINITY : h);
+        }
+      if (dem[p] == lowest) {
+        out[p] |= 1;
+        flats++;
+      }
+    }
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (out[p] & 1) continue;
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          if (!di && !dj) continue;
+          const ptrdiff_t ii = i + di, jj = j + dj;
+          if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+          const ptrdiff_t q = jj * nx + ii;
+          if ((out[q] & 1) && dem[q] == dem[p]) {
+            out[p] |= 2;
+            out[q] |= 4;
+          }
+        }
+    }
+  return flats;
+}
+
+/* Minimum (op < 0) or maximum (op > 0) over the cells under a structuring
+   element of se_dims[0] x se_dims[1] x se_dims[2] (slices visited in turn)
+   centred at ((se_dims[0] + 1) / 2 - 1, (se_dims[1] + 1) / 2 - 1); NaN
+   cells stay NaN, NaN neighbours and cells outside the grid are skipped,
+   the running value is replaced unless it is strictly better (so equal
+   values, e.g. -0 and +0, take the later one). src/value_filters.c:9-113. */
+void oracle_value_filter(float *out, const float *dem, const uint8_t *se,
+                         const ptrdiff_t dims[2], const ptrdiff_t se_dims[3], int op) {
+  const ptrdiff_t nx = dims[0], ny = dims[1];
+  const ptrdiff_t cx = (se_dims[0] + 1) / 2 - 1, cy = (se_dims[1] + 1) / 2 - 1;
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (isnan(dem[p])) {
+        out[p] = NAN;
+        continue;
+      }
+      float v = op < 0 ? INFINITY : -INFINITY;
+      for (ptrdiff_t s = 0; s < se_dims[2]; s++)
+        for (ptrdiff_t b = 0; b < se_dims[1]; b++) {
+          const ptrdiff_t jj = j + b - cy;
+          if (jj < 0 || jj >= ny) continue;
+          for (ptrdiff_t a = 0; a < se_dims[0]; a++) {
+            if (!se[a + b * se_dims[0] + s * se_dims[0] * se_dims[1]]) continue;
+            const ptrdiff_t ii = i + a - cx;
+            if (ii < 0 || ii >= nx) continue;
+            const float h = dem[jj * nx + ii];
+            if (isnan(h)) continue;
+            if (op < 0 ? (v < h) : (v > h)) continue;
+            v = h;
+          }
+        }
+      out[p] = v;
+    }
+}
+
+/* Square, full structuring element of the given width, separably: first
+   along dims[0] into tmp (NaN inputs skipped), then along dims[1]
+   (NaN cells of the input stay NaN). src/value_filters.c:115-246. */
+void oracle_value_filter_square(float *out, const float *dem, float *tmp, int width,
+                                const ptrdiff_t dims[2], int op) {
+  const ptrdiff_t nx = dims[0], ny = dims[1], c = (width + 1) / 2 - 1;
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      float v = op < 0 ? INFINITY : -INFINITY;
+      for (ptrdiff_t k = 0; k < width; k++) {
+        const ptrdiff_t ii = i + k - c;
+        if (ii < 0 || ii >= nx) continue;
+        const float h = dem[j * nx + ii];
+        if (isnan(h)) continue;
+        if (op < 0 ? (v < h) : (v > h)) continue;
+        v = h;
+      }
+      tmp[j * nx + i] = v;
+    }
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (isnan(dem[p])) {
+        out[p] = NAN;
+        continue;
+      }
+      float v = op < 0 ? INFINITY : -INFINITY;
+      for (ptrdiff_t k = 0; k < width; k++) {
+        const ptrdiff_t jj = j + k - c;
+        if (jj < 0 || jj >= ny) continue;
+        const float h = tmp[jj * nx + i];
+        if (isnan(h)) continue;
+        if (op < 0 ? (v < h) : (v > h)) continue;
+        v = h;
+      }
+      out[p] = v;
+    }
+}
+

@@ -336,3 +336,89 @@ def test_fsm_adjoint_fd(oracle_lib, seed):
         fd_s = (L(dem, s + e) - L(dem, s - e)) / (2 * eps)
         assert abs(fd_d - g_dem[j, i]) < 1e-5 * max(1.0, abs(fd_d)), ("dem", j, i, fd_d, g_dem[j, i])
         assert abs(fd_s - g_s[j, i]) < 1e-5 * max(1.0, abs(fd_s)), ("slope", j, i, fd_s, g_s[j, i])
+
+
+# ------------------------------------------------------------------ flats and value filters
+def stencil_dem(rng, shape, flats=True):
+    """Random DEM with plateaus, NaN, +-INF and -0.0 cells."""
+    dem = rng.uniform(0, 50, size=shape).astype(np.float32)
+    if flats:
+        dem = np.round(dem / 5) * 5
+    dem[rng.uniform(size=shape) < 0.03] = np.nan
+    dem[rng.uniform(size=shape) < 0.01] = np.inf
+    dem[rng.uniform(size=shape) < 0.01] = -np.inf
+    zeros = rng.uniform(size=shape) < 0.05
+    dem[zeros] = np.where(rng.uniform(size=zeros.sum()) < 0.5, -0.0, 0.0)
+    return dem
+
+
+def test_identifyflats_vs_reference_and_invariants(oracle_lib):
+    rng = np.random.default_rng(61)
+    for shape in [(1, 1), (3, 3), (1, 40), (40, 1), (57, 83)]:
+        dem = stencil_dem(rng, shape)
+        out, n = oracle_lib.identifyflats(dem)
+        if oracle_lib.ref() is not None:
+            want, wn = oracle_lib.ref_identifyflats(dem)
+            np.testing.assert_array_equal(out, want)
+            assert n == wn
+        assert n == int((out & 1).sum())
+        ny, nx = shape
+        for j in range(ny):
+            for i in range(nx):
+                nb = [(j + a, i + b) for a in (-1, 0, 1) for b in (-1, 0, 1)
+                      if (a or b) and 0 <= j + a < ny and 0 <= i + b < nx]
+                if out[j, i] & 1:  # flat: interior, nothing strictly lower
+                    assert 0 < j < ny - 1 and 0 < i < nx - 1
+                    assert all(not (dem[q] < dem[j, i]) and not np.isnan(dem[q]) or
+                               dem[j, i] == -np.inf for q in nb)
+                if out[j, i] & 2:  # sill: not flat, touches an equal flat
+                    assert not out[j, i] & 1
+                    assert any(out[q] & 1 and dem[q] == dem[j, i] for q in nb)
+                if out[j, i] & 4:  # presill: a flat touching an equal non-flat
+                    assert out[j, i] & 1
+                    assert any(not out[q] & 1 and dem[q] == dem[j, i] for q in nb)
+
+
+def test_value_filters_vs_reference(oracle_lib):
+    if oracle_lib.ref() is None:
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(62)
+    ses = [np.ones((3, 3)), np.array([[0, 1, 0], [1, 1, 1], [0, 1, 0]]), np.ones((1, 1)),
+           np.ones((2, 4)), rng.uniform(size=(2, 5, 3)) < 0.5]
+    for shape in [(1, 1), (2, 7), (45, 38)]:
+        dem = stencil_dem(rng, shape, flats=False)
+        for se in ses:
+            for op in ("min", "max"):
+                assert_bitwise(oracle_lib.value_filter(dem, se, op),
+                               oracle_lib.ref_value_filter(dem, se, op), f"{op} {se.shape}")
+        for width in (1, 2, 3, 5, 8):
+            for op in ("min", "max"):
+                assert_bitwise(oracle_lib.value_filter_square(dem, width, op),
+                               oracle_lib.ref_value_filter_square(dem, width, op),
+                               f"square {op} {width}")
+
+
+def test_value_filter_mathematics(oracle_lib):
+    """Pins that do not need the reference: on NaN-free inputs without
+    signed zeros, the square filter equals the general filter with a full
+    square structuring element (separability), max is -min(-x) (duality),
+    a 1x1 element is the identity and the result brackets the input."""
+    rng = np.random.default_rng(63)
+    dem = rng.uniform(-20, 50, size=(31, 44)).astype(np.float32) + np.float32(0.0)
+    for width in (1, 3, 4, 7):
+        se = np.ones((width, width), np.uint8)
+        for op in ("min", "max"):
+            assert_bitwise(oracle_lib.value_filter_square(dem, width, op),
+                           oracle_lib.value_filter(dem, se, op), f"separable {op} {width}")
+        assert_bitwise(oracle_lib.value_filter(dem, se, "max"),
+                       -oracle_lib.value_filter(-dem, se, "min"), f"duality {width}")
+    assert_bitwise(oracle_lib.value_filter(dem, np.ones((1, 1)), "min"), dem, "identity")
+    lo = oracle_lib.value_filter(dem, np.ones((3, 3)), "min")
+    hi = oracle_lib.value_filter(dem, np.ones((3, 3)), "max")
+    assert (lo <= dem).all() and (hi >= dem).all()
+    # brute force on a tiny grid: window minimum of the cells in range
+    small = dem[:6, :7]
+    got = oracle_lib.value_filter(small, np.ones((3, 3)), "min")
+    for j in range(6):
+        for i in range(7):
+            assert got[j, i] == small[max(j - 1, 0):j + 2, max(i - 1, 0):i + 2].min()
