@@ -16,7 +16,9 @@ Prints one JSON line. `value` is Gcells/s of the selected workload;
 algorithmic bytes (fill 9 B/cell: dem + bc + output; fsm 12 B/cell: dem +
 slopes + output) and 8 TB/s; `cpu_baseline` times the reference path on
 the host (one core) on a bounded crop of the same DEM. The other workload
-is measured too and reported under "extra".
+is measured too and reported under "extra", with the forward+adjoint
+passes, the batched launch and the stencils next to the path
+(identifyflats, min / max filters).
 """
 from __future__ import annotations
 
@@ -272,6 +274,68 @@ def batched_run(op: str, count: int, steps: int):
     return res
 
 
+# Stencils next to the path, bytes per cell: identifyflats reads the DEM and
+# writes int32 flags; a value filter reads the DEM and writes the result; the
+# square filter's contract also writes tmp (the pass along dims[0]).
+STENCIL_BYTES = {"identifyflats": 8, "min_filter_3x3": 8, "max_filter_square_5": 12}
+
+
+def stencil_run(n: int, steps: int, cpu: bool):
+    """identifyflats, min_filter (3x3) and max_filter_square (width 5) on
+    the filled fill-workload DEM (n x n): launch time from HIP events, the
+    roofline against the bytes above, and the reference's CPU routine on a
+    1024 x 1024 crop."""
+    import torch
+
+    import libtopotoolbox_amd as tt
+
+    arrays = make_inputs("fill", n)
+    dev = torch.device("cuda:0")
+    dem = tt.fillsinks(torch.from_numpy(arrays["dem"]).to(dev),
+                       torch.from_numpy(arrays["bc"]).to(dev))
+    del arrays
+    se = torch.ones((3, 3), dtype=torch.uint8, device=dev)
+    out = torch.empty_like(dem)
+    tmp = torch.empty_like(dem)
+    calls = {
+        "identifyflats": lambda: tt.identifyflats(dem, count=False),
+        "min_filter_3x3": lambda: tt.min_filter(dem, se, out=out),
+        "max_filter_square_5": lambda: tt.max_filter_square(dem, 5, out=out, tmp=tmp),
+    }
+    res = {}
+    crop = None
+    if cpu:
+        import oracle
+
+        crop = np.ascontiguousarray(dem[:1024, :1024].cpu().numpy())
+    for name, fn in calls.items():
+        fn()
+        _, per = time_events(fn, steps)
+        ms = float(np.mean(per))
+        gbs = STENCIL_BYTES[name] * n * n / (ms * 1e-3) / 1e9
+        r = {"ms": round(ms, 4), "gcells_per_s": round(n * n / (ms * 1e-3) / 1e9, 2),
+             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "bytes_per_cell": STENCIL_BYTES[name]}}
+        if crop is not None and oracle.ref() is not None:
+            ref = {"identifyflats": lambda: oracle.ref_identifyflats(crop),
+                   "min_filter_3x3": lambda: oracle.ref_value_filter(
+                       crop, np.ones((3, 3), np.uint8), "min"),
+                   "max_filter_square_5": lambda: oracle.ref_value_filter_square(
+                       crop, 5, "max")}[name]
+            t0 = time.perf_counter()
+            for _ in range(5):
+                ref()
+            dt = (time.perf_counter() - t0) / 5
+            r["cpu_baseline"] = {"value": round(crop.size / dt / 1e9, 4), "unit": "Gcells/s",
+                                 "cores": 1, "kind": "reference",
+                                 "sample": "1024x1024 crop of the filled DEM, 5 calls"}
+        res[name] = r
+    del dem, out, tmp
+    torch.cuda.empty_cache()
+    return res
+
+
 def roofline(workload: str, cells: int, launch_ms: float, traffic):
     algo = BYTES_PER_CELL[workload] * cells
     achieved = algo / (launch_ms * 1e-3) / 1e9
@@ -334,6 +398,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-stencils", action="store_true")
     ap.add_argument("--batch-tiles", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size "
@@ -436,6 +501,9 @@ def main():
             extra.setdefault("fwd_adjoint", {})[other] = fwd_adjoint_run(
                 other, a2, max(3, args.steps // 2), float(np.mean(r2["launch_ms"])))
         del a2, r2
+    if not args.no_stencils:
+        extra["stencils"] = stencil_run(n, max(5, args.steps), not args.no_cpu)
+        print(f"[bench] stencils: {json.dumps(extra['stencils'])}", file=sys.stderr, flush=True)
     if not args.no_batched:
         import torch
 

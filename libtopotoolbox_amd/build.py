@@ -52,17 +52,37 @@ def sources() -> list:
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile the library (always) and return its path."""
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-I", os.path.join(ROOT, "include"), *sources(), "-o", tmp]
+def _run(cmd: list, verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}: {' '.join(cmd)}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the library (always) and return its path. Each source is
+    compiled to an object in parallel (every translation unit carries its
+    own device code), then the objects are linked into the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OUT_DIR, exist_ok=True)
+    obj_dir = os.path.join(OUT_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    jobs = []
+    for src in sources():
+        obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
+        jobs.append((obj, [hipcc(), *compile_flags, *inc, "-c", src, "-o", obj]))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd, verbose) for _, cmd in jobs]:
+            f.result()
+    tmp = LIB_PATH + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *[o for o, _ in jobs], "-o",
+          tmp], verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -1,168 +1,450 @@
 // Single-pass stencils that consume a DEM next to the hot path: flat / sill
 // / presill classification and the minimum / maximum value filters. Each
 // output cell depends only on a fixed neighbourhood of the input, so these
-// are one read and one write per cell, streamed at HBM rate.
+// are one read and one write per cell at HBM rate: a workgroup stages a
+// window (its tile plus the stencil's halo) into LDS with coalesced row
+// loads, and every cell folds its neighbourhood out of LDS.
 //
 // Reference semantics:
 //   identifyflats      src/identifyflats.c:32-121
-//   min_filter,        src/value_filters.c:9-113: a sequential fold over
+//   min_filter,        src/value_filters.c:9-110: a sequential fold over
 //   max_filter         the structuring element (slices, rows, columns in
 //                      that order), the running value replaced unless it is
 //                      strictly better, so the result's bits (e.g. -0 vs
 //                      +0) follow the same order here
-//   *_filter_square    src/value_filters.c:115-246: the same fold along
+//   *_filter_square    src/value_filters.c:112-246: the same fold along
 //                      dims[0] into tmp, then along dims[1]
+//
+// Cells outside the grid are staged as NaN: the filters skip NaN
+// neighbours and out-of-grid neighbours alike, and identifyflats never
+// matches a NaN, so one marker serves both.
 #include "kernels.h"
 
 namespace tt {
 
-// ---------------------------------------------------------------- identifyflats
-constexpr int kFlatTW = 64, kFlatTH = 8, kFlatThreads = 256;
+// Tiles are kTW cells wide (one column per lane and kCols columns per lane)
+// and kTH rows tall; four waves share the rows.
+constexpr int kCols = 4, kTW = 64 * kCols, kTH = 16, kWaves = 4, kThreads = 64 * kWaves;
 
-// Tile of kFlatTW x kFlatTH cells; LDS holds the DEM with a 2-cell halo
-// and the flat bits with a 1-cell halo (a sill or presill of a tile cell
-// depends on its neighbours being flats).
-__global__ __launch_bounds__(kFlatThreads) void identifyflats_kernel(
-    int32_t* out, const float* dem, long long nx, long long ny, unsigned long long* count) {
-  constexpr int DW = kFlatTW + 4, DH = kFlatTH + 4;  // DEM window
-  constexpr int FW = kFlatTW + 2, FH = kFlatTH + 2;  // flat-bit window
-  __shared__ float sd[DH][DW];
-  __shared__ unsigned char sflat[FH][FW];
-  __shared__ unsigned block_flats;
-  const int tid = threadIdx.x;
-  const long long ntx = (nx + kFlatTW - 1) / kFlatTW, nty = (ny + kFlatTH - 1) / kFlatTH;
-  for (long long t = blockIdx.x; t < ntx * nty; t += gridDim.x) {
-    const long long x0 = (t % ntx) * kFlatTW, y0 = (t / ntx) * kFlatTH;
-    if (tid == 0) block_flats = 0u;
-    for (int k = tid; k < DW * DH; k += kFlatThreads) {
-      const int r = k / DW, c = k % DW;
-      const long long gx = x0 + c - 2, gy = y0 + r - 2;
-      sd[r][c] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? dem[gy * nx + gx]
-                                                            : __builtin_nanf("");
-    }
-    __syncthreads();
-    // flats: interior cells equal to the minimum of their 3x3 window, NaN
-    // neighbours counting as -INF
-    for (int k = tid; k < FW * FH; k += kFlatThreads) {
-      const int r = k / FW, c = k % FW;  // DEM window cell (r + 1, c + 1)
-      const long long gx = x0 + c - 1, gy = y0 + r - 1;
-      bool flat = false;
-      if (gx > 0 && gx < nx - 1 && gy > 0 && gy < ny - 1) {
-        const float h = sd[r + 1][c + 1];
-        float lowest = h;
-        for (int dr = 0; dr < 3; dr++)
-          for (int dc = 0; dc < 3; dc++) {
-            if (dr == 1 && dc == 1) continue;
-            const float q = sd[r + dr][c + dc];
-            lowest = fminf(lowest, q != q ? -__builtin_inff() : q);
-          }
-        flat = h == lowest;
+__device__ __forceinline__ float nan_marker() { return __builtin_nanf(""); }
+
+// A window of up to RPW rows per wave and kChunks x 64 columns, loaded
+// into registers (all loads of a thread in flight together) and then
+// written to LDS. The tiled kernels load the next tile's window while they
+// compute the current one, so HBM latency overlaps the LDS work.
+constexpr int kChunks = 5;
+
+template <int RPW>
+struct WindowLoader {
+  float v[RPW][kChunks];
+
+  // rows [wy0, wy0 + H) x columns [wx0, wx0 + W) of src; NaN outside the grid
+  __device__ __forceinline__ void load(const float* src, long long nx, long long ny,
+                                       long long wx0, long long wy0, int W, int H) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int u = 0; u < RPW; u++) {
+      const int r = wave + kWaves * u;
+      const long long gy = wy0 + r;
+      const bool row_ok = r < H && gy >= 0 && gy < ny;
+#pragma unroll
+      for (int ch = 0; ch < kChunks; ch++) {
+        const int c = lane + 64 * ch;
+        const long long gx = wx0 + c;
+        v[u][ch] = (row_ok && c < W && gx >= 0 && gx < nx) ? src[gy * nx + gx] : nan_marker();
       }
-      sflat[r][c] = flat ? 1 : 0;
     }
+  }
+
+  // into LDS s with row pitch P
+  __device__ __forceinline__ void store(float* s, int P, int W, int H) const {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int u = 0; u < RPW; u++) {
+      const int r = wave + kWaves * u;
+#pragma unroll
+      for (int ch = 0; ch < kChunks; ch++) {
+        const int c = lane + 64 * ch;
+        if (r < H && c < W) s[r * P + c] = v[u][ch];
+      }
+    }
+  }
+};
+
+// Tiles of the grid in row-major order, split into contiguous chunks per
+// XCD (workgroups are dispatched round-robin over the 8 XCDs, each with
+// its own L2): vertically and horizontally adjacent tiles, which share
+// halo rows and cache lines, then tend to run on the same XCD.
+struct TileWalk {
+  long long ntx, ntiles, per_xcd, begin, end, stride;
+  __device__ TileWalk(long long nx, long long ny) {
+    ntx = (nx + kTW - 1) / kTW;
+    ntiles = ntx * ((ny + kTH - 1) / kTH);
+    const int xcds = gridDim.x >= 8 ? 8 : 1;
+    const long long xcd = blockIdx.x % xcds, rank = blockIdx.x / xcds;
+    const long long groups = gridDim.x / xcds;  // workgroups per XCD
+    per_xcd = (ntiles + xcds - 1) / xcds;
+    begin = xcd * per_xcd + rank;
+    end = (xcd + 1) * per_xcd < ntiles ? (xcd + 1) * per_xcd : ntiles;
+    stride = groups;
+    // workgroups beyond groups * xcds (grid not a multiple of 8) idle
+    if (rank >= groups) begin = end;
+  }
+};
+
+// ---------------------------------------------------------------- identifyflats
+// Window: the tile plus 2 cells on every side (a sill or presill of a tile
+// cell depends on whether its neighbours are flats, which depends on their
+// neighbours). Flat bits are computed for the tile plus 1 cell.
+constexpr int kFlatW = kTW + 4, kFlatH = kTH + 4;
+constexpr int kFlagW = kTW + 2, kFlagH = kTH + 2;
+
+__global__ __launch_bounds__(kThreads) void identifyflats_kernel(
+    int32_t* out, const float* dem, long long nx, long long ny, unsigned long long* partial) {
+  __shared__ float sd[kFlatH * kFlatW];
+  __shared__ unsigned char sflat[kFlagH * kFlagW];
+  __shared__ unsigned long long wave_count[kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const TileWalk w(nx, ny);
+  unsigned long long flats = 0;
+  WindowLoader<(kFlatH + kWaves - 1) / kWaves> win;
+  if (w.begin < w.end)
+    win.load(dem, nx, ny, (w.begin % w.ntx) * kTW - 2, (w.begin / w.ntx) * kTH - 2, kFlatW,
+             kFlatH);
+  for (long long t = w.begin; t < w.end; t += w.stride) {
+    const long long x0 = (t % w.ntx) * kTW, y0 = (t / w.ntx) * kTH;
+    win.store(sd, kFlatW, kFlatW, kFlatH);
     __syncthreads();
-    unsigned flats = 0;
-    for (int k = tid; k < kFlatTW * kFlatTH; k += kFlatThreads) {
-      const int r = k / kFlatTW, c = k % kFlatTW;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx >= nx || gy >= ny) continue;
-      const bool flat = sflat[r + 1][c + 1];
-      const float h = sd[r + 2][c + 2];
-      int32_t bits = flat ? 1 : 0;
-      for (int dr = -1; dr <= 1; dr++)
-        for (int dc = -1; dc <= 1; dc++) {
-          if (!dr && !dc) continue;
-          const long long qx = gx + dc, qy = gy + dr;
-          if (qx < 0 || qx >= nx || qy < 0 || qy >= ny) continue;
-          if (sd[r + 2 + dr][c + 2 + dc] != h) continue;  // NaN never equal
-          const bool qflat = sflat[r + 1 + dr][c + 1 + dc];
-          if (!flat && qflat) bits |= 2;  // sill: touches an equal flat
-          if (flat && !qflat) bits |= 4;  // presill: a flat touching an equal sill
+    const long long tn = t + w.stride;
+    if (tn < w.end)
+      win.load(dem, nx, ny, (tn % w.ntx) * kTW - 2, (tn / w.ntx) * kTH - 2, kFlatW, kFlatH);
+    // flats: interior cells equal to the minimum of their 3x3 window, NaN
+    // neighbours counting as -INF (a NaN cell is never equal to anything)
+    for (int r = wave; r < kFlagH; r += kWaves) {
+      const long long gy = y0 - 1 + r;
+      const bool row_in = gy > 0 && gy < ny - 1;
+      for (int c = lane; c < kFlagW; c += 64) {
+        const long long gx = x0 - 1 + c;
+        bool flat = false;
+        if (row_in && gx > 0 && gx < nx - 1) {
+          const float* q = sd + r * kFlatW + c;  // window cell (r + 1, c + 1) is the centre
+          const float h = q[kFlatW + 1];
+          float lowest = h;
+#pragma unroll
+          for (int dr = 0; dr < 3; dr++)
+#pragma unroll
+            for (int dc = 0; dc < 3; dc++) {
+              if (dr == 1 && dc == 1) continue;
+              const float v = q[dr * kFlatW + dc];
+              lowest = fminf(lowest, v != v ? -__builtin_inff() : v);
+            }
+          flat = h == lowest;
         }
-      out[gy * nx + gx] = bits;
-      flats += flat;
+        sflat[r * kFlagW + c] = flat;
+      }
     }
-    // wave sum, then one atomic per wave into the tile's count
-    for (int o = 32; o > 0; o >>= 1) flats += __shfl_xor(flats, o);
-    if ((tid & 63) == 0 && flats) atomicAdd(&block_flats, flats);
     __syncthreads();
-    if (tid == 0 && block_flats) atomicAdd(count, (unsigned long long)block_flats);
+    for (int r = wave; r < kTH; r += kWaves) {
+      const long long gy = y0 + r;
+      if (gy >= ny) break;
+#pragma unroll
+      for (int k = 0; k < kCols; k++) {
+        const int c = lane + 64 * k;
+        const long long gx = x0 + c;
+        if (gx >= nx) continue;
+        const bool flat = sflat[(r + 1) * kFlagW + c + 1];
+        const float h = sd[(r + 2) * kFlatW + c + 2];
+        int32_t bits = flat;
+#pragma unroll
+        for (int dr = -1; dr <= 1; dr++)
+#pragma unroll
+          for (int dc = -1; dc <= 1; dc++) {
+            if (!dr && !dc) continue;
+            // out-of-grid neighbours are NaN in the window and never equal
+            if (sd[(r + 2 + dr) * kFlatW + c + 2 + dc] != h) continue;
+            const bool qflat = sflat[(r + 1 + dr) * kFlagW + c + 1 + dc];
+            if (!flat && qflat) bits |= 2;  // sill: touches an equal flat
+            if (flat && !qflat) bits |= 4;  // presill: a flat touching an equal sill
+          }
+        out[gy * nx + gx] = bits;
+        flats += flat;
+      }
+    }
+    __syncthreads();  // the next tile's store overwrites sd and sflat
+  }
+  if (partial) {
+    for (int o = 32; o > 0; o >>= 1) flats += __shfl_xor(flats, o);
+    if (lane == 0) wave_count[wave] = flats;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long s = 0;
+      for (int i = 0; i < kWaves; i++) s += wave_count[i];
+      partial[blockIdx.x] = s;
+    }
+  }
+}
+
+__global__ void sum_partials(unsigned long long* partial, int n) {
+  __shared__ unsigned long long s[256];
+  unsigned long long v = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
     __syncthreads();
   }
+  if (threadIdx.x == 0) partial[n] = s[0];
 }
 
 // ---------------------------------------------------------------- value filters
-// op < 0: minimum, op > 0: maximum. `better(v, h)`: v stays unless h beats
-// it strictly (the reference's "continue" test), so ties take h.
+// op < 0: minimum, op > 0: maximum. The running value v stays only if it
+// beats h strictly (the reference's "continue" test), so ties take h; NaN
+// (and out-of-grid) cells are skipped.
 template <int OP>
 __device__ __forceinline__ float fold(float v, float h) {
-  if (h != h) return v;
-  return (OP < 0 ? (v < h) : (v > h)) ? v : h;
+  return (h != h || (OP < 0 ? (v < h) : (v > h))) ? v : h;
 }
 
-template <int OP>
-__global__ void value_filter_kernel(float* out, const float* dem, const uint8_t* se, long long sx,
-                                    long long sy, long long ss, long long nx, long long ny) {
-  const long long cx = (sx + 1) / 2 - 1, cy = (sy + 1) / 2 - 1;
-  const long long n = nx * ny;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const long long i = p % nx, j = p / nx;
-    const float self = dem[p];
-    if (self != self) {
-      out[p] = __builtin_nanf("");
-      continue;
+// Structuring elements handled through LDS: at most kSeMax set entries and
+// a halo of at most kHaloMax cells on each axis; larger ones take the
+// direct kernel below.
+constexpr int kSeMax = 1024, kHaloMax = 16;
+
+// The structuring element's set entries in the reference's visiting order
+// (slice, row, column), packed as (row << 16 | column); returns the count.
+__device__ int compact_se(unsigned* list, const uint8_t* se, long long sx, long long sy,
+                          long long ss) {
+  __shared__ unsigned wave_base[kWaves];
+  __shared__ unsigned total;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long n = sx * sy * ss;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  for (long long k0 = 0; k0 < n; k0 += kThreads) {
+    const long long k = k0 + threadIdx.x;
+    const bool set = k < n && se[k];
+    const unsigned long long m = __ballot(set);
+    const unsigned below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_base[wave] = (unsigned)__popcll(m);
+    __syncthreads();
+    unsigned off = total;
+    for (int i = 0; i < wave; i++) off += wave_base[i];
+    if (set) {
+      const long long in_slice = k % (sx * sy);
+      list[off + below] = (unsigned)((in_slice / sx) << 16 | (in_slice % sx));
     }
-    float v = OP < 0 ? __builtin_inff() : -__builtin_inff();
-    for (long long s = 0; s < ss; s++)
-      for (long long b = 0; b < sy; b++) {
-        const long long jj = j + b - cy;
-        if (jj < 0 || jj >= ny) continue;
-        const uint8_t* row = se + (s * sy + b) * sx;
-        for (long long a = 0; a < sx; a++) {
-          if (!row[a]) continue;
-          const long long ii = i + a - cx;
-          if (ii < 0 || ii >= nx) continue;
-          v = fold<OP>(v, dem[jj * nx + ii]);
-        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < kWaves; i++) total += wave_base[i];
+    __syncthreads();
+  }
+  return (int)total;
+}
+
+template <int OP, int RPW>
+__global__ __launch_bounds__(kThreads) void value_filter_tiled(
+    float* out, const float* dem, const uint8_t* se, long long sx, long long sy, long long ss,
+    long long nx, long long ny) {
+  extern __shared__ float smem[];
+  __shared__ unsigned list[kSeMax];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cx = (int)((sx + 1) / 2 - 1), cy = (int)((sy + 1) / 2 - 1);
+  const int W = kTW + (int)sx - 1, H = kTH + (int)sy - 1;
+  const TileWalk w(nx, ny);
+  WindowLoader<RPW> win;
+  if (w.begin < w.end)
+    win.load(dem, nx, ny, (w.begin % w.ntx) * kTW - cx, (w.begin / w.ntx) * kTH - cy, W, H);
+  const int nlist = compact_se(list, se, sx, sy, ss);
+  for (long long t = w.begin; t < w.end; t += w.stride) {
+    const long long x0 = (t % w.ntx) * kTW, y0 = (t / w.ntx) * kTH;
+    win.store(smem, W, W, H);
+    __syncthreads();
+    const long long tn = t + w.stride;
+    if (tn < w.end) win.load(dem, nx, ny, (tn % w.ntx) * kTW - cx, (tn / w.ntx) * kTH - cy, W, H);
+    for (int r = wave; r < kTH; r += kWaves) {
+      const long long gy = y0 + r;
+      if (gy >= ny) break;
+      float v[kCols];
+#pragma unroll
+      for (int k = 0; k < kCols; k++) v[k] = OP < 0 ? __builtin_inff() : -__builtin_inff();
+      for (int e = 0; e < nlist; e++) {
+        const unsigned ab = list[e];
+        const float* row = smem + (r + (int)(ab >> 16)) * W + (int)(ab & 0xffffu) + lane;
+#pragma unroll
+        for (int k = 0; k < kCols; k++) v[k] = fold<OP>(v[k], row[64 * k]);
       }
-    out[p] = v;
+#pragma unroll
+      for (int k = 0; k < kCols; k++) {
+        const int c = lane + 64 * k;
+        const long long gx = x0 + c;
+        if (gx >= nx) continue;
+        const float self = smem[(r + cy) * W + c + cx];
+        out[gy * nx + gx] = self != self ? nan_marker() : v[k];
+      }
+    }
+    __syncthreads();
   }
 }
 
-// One axis of the square filter: along x (AXIS 0, from `in` into `out`,
-// no NaN pass-through) or along y (AXIS 1, NaN cells of `dem` stay NaN).
+// Any structuring element: each cell folds straight from global memory.
+template <int OP>
+__global__ void value_filter_direct(float* out, const float* dem, const uint8_t* se,
+                                    long long sx, long long sy, long long ss, long long nx,
+                                    long long ny) {
+  const long long cx = (sx + 1) / 2 - 1, cy = (sy + 1) / 2 - 1;
+  for (long long j = blockIdx.y; j < ny; j += gridDim.y)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nx;
+         i += (long long)gridDim.x * blockDim.x) {
+      const long long p = j * nx + i;
+      if (dem[p] != dem[p]) {
+        out[p] = nan_marker();
+        continue;
+      }
+      float v = OP < 0 ? __builtin_inff() : -__builtin_inff();
+      for (long long s = 0; s < ss; s++)
+        for (long long b = 0; b < sy; b++) {
+          const long long jj = j + b - cy;
+          if (jj < 0 || jj >= ny) continue;
+          const uint8_t* row = se + (s * sy + b) * sx;
+          for (long long a = 0; a < sx; a++) {
+            const long long ii = i + a - cx;
+            if (!row[a] || ii < 0 || ii >= nx) continue;
+            v = fold<OP>(v, dem[jj * nx + ii]);
+          }
+        }
+      out[p] = v;
+    }
+}
+
+// Square filter in one pass: each window row is folded along x and the
+// result written over the row's first kTW entries in LDS (one wave owns a
+// row, and its lanes read the row before any of them writes it). Rows
+// outside the grid are marked NaN, which the vertical fold skips; tmp
+// itself is never NaN, since each fold starts at +-INF and skips NaN. The
+// tile's own rows also go to `tmp`; whether a tile cell is NaN is kept
+// aside before its row is overwritten.
+constexpr int kSquareMax = 33;
+
+template <int OP, int RPW>
+__global__ __launch_bounds__(kThreads) void value_filter_square_tiled(
+    float* out, float* tmp, const float* dem, int width, long long nx, long long ny) {
+  extern __shared__ float smem[];
+  __shared__ unsigned char self_nan[kTH * kTW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = (width + 1) / 2 - 1;
+  const int W = kTW + width - 1, H = kTH + width - 1;
+  const TileWalk w(nx, ny);
+  WindowLoader<RPW> win;
+  if (w.begin < w.end)
+    win.load(dem, nx, ny, (w.begin % w.ntx) * kTW - c, (w.begin / w.ntx) * kTH - c, W, H);
+  for (long long t = w.begin; t < w.end; t += w.stride) {
+    const long long x0 = (t % w.ntx) * kTW, y0 = (t / w.ntx) * kTH;
+    win.store(smem, W, W, H);
+    __syncthreads();
+    const long long tn = t + w.stride;
+    if (tn < w.end) win.load(dem, nx, ny, (tn % w.ntx) * kTW - c, (tn / w.ntx) * kTH - c, W, H);
+    for (int r = wave; r < H; r += kWaves) {
+      const long long gy = y0 - c + r;
+      const bool row_in = gy >= 0 && gy < ny;
+      const bool own = r >= c && r < c + kTH;
+      float* row = smem + r * W;
+      float v[kCols];
+#pragma unroll
+      for (int k = 0; k < kCols; k++) v[k] = OP < 0 ? __builtin_inff() : -__builtin_inff();
+      for (int a = 0; a < width; a++)
+#pragma unroll
+        for (int k = 0; k < kCols; k++) v[k] = fold<OP>(v[k], row[lane + a + 64 * k]);
+      if (own) {
+#pragma unroll
+        for (int k = 0; k < kCols; k++) {
+          const float self = row[lane + 64 * k + c];
+          self_nan[(r - c) * kTW + lane + 64 * k] = self != self;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kCols; k++) {
+        const int col = lane + 64 * k;
+        row[col] = row_in ? v[k] : nan_marker();
+        if (own && row_in && x0 + col < nx) tmp[gy * nx + x0 + col] = v[k];
+      }
+    }
+    __syncthreads();
+    for (int r = wave; r < kTH; r += kWaves) {
+      const long long gy = y0 + r;
+      if (gy >= ny) break;
+      float v[kCols];
+#pragma unroll
+      for (int k = 0; k < kCols; k++) v[k] = OP < 0 ? __builtin_inff() : -__builtin_inff();
+      for (int b = 0; b < width; b++)
+#pragma unroll
+        for (int k = 0; k < kCols; k++) v[k] = fold<OP>(v[k], smem[(r + b) * W + lane + 64 * k]);
+#pragma unroll
+      for (int k = 0; k < kCols; k++) {
+        const int col = lane + 64 * k;
+        if (x0 + col >= nx) continue;
+        out[gy * nx + x0 + col] = self_nan[r * kTW + col] ? nan_marker() : v[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One axis of the square filter for any width: along x (AXIS 0, from `in`
+// into `out`, no NaN pass-through) or along y (AXIS 1, NaN cells of `dem`
+// stay NaN).
 template <int OP, int AXIS>
 __global__ void value_filter_axis(float* out, const float* in, const float* dem, int width,
                                   long long nx, long long ny) {
   const long long c = (width + 1) / 2 - 1;
-  const long long n = nx * ny;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const long long i = p % nx, j = p / nx;
-    if (AXIS == 1 && dem[p] != dem[p]) {
-      out[p] = __builtin_nanf("");
-      continue;
-    }
-    float v = OP < 0 ? __builtin_inff() : -__builtin_inff();
-    for (int k = 0; k < width; k++) {
-      if (AXIS == 0) {
-        const long long ii = i + k - c;
-        if (ii >= 0 && ii < nx) v = fold<OP>(v, in[j * nx + ii]);
-      } else {
-        const long long jj = j + k - c;
-        if (jj >= 0 && jj < ny) v = fold<OP>(v, in[jj * nx + i]);
+  for (long long j = blockIdx.y; j < ny; j += gridDim.y)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nx;
+         i += (long long)gridDim.x * blockDim.x) {
+      const long long p = j * nx + i;
+      if (AXIS == 1 && dem[p] != dem[p]) {
+        out[p] = nan_marker();
+        continue;
       }
+      float v = OP < 0 ? __builtin_inff() : -__builtin_inff();
+      for (int k = 0; k < width; k++) {
+        if (AXIS == 0) {
+          const long long ii = i + k - c;
+          if (ii >= 0 && ii < nx) v = fold<OP>(v, in[j * nx + ii]);
+        } else {
+          const long long jj = j + k - c;
+          if (jj >= 0 && jj < ny) v = fold<OP>(v, in[jj * nx + i]);
+        }
+      }
+      out[p] = v;
     }
-    out[p] = v;
-  }
 }
 
-__global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+// ---------------------------------------------------------------- host side
+// Workgroups for a tiled kernel: enough to fill every CU at the kernel's
+// occupancy, a multiple of the 8 XCDs, no more than the tiles.
+static hipError_t tiled_grid(const void* kernel, size_t lds, long long ntiles, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, lds);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) return hipErrorInvalidConfiguration;
+  long long g = (long long)cus * per_cu * 2;  // two rounds of tiles in flight
+  if (g > ntiles) g = ntiles;
+  if (g >= 8) g -= g % 8;
+  *grid = (int)(g < 1 ? 1 : g);
+  return hipSuccess;
+}
 
-static int stream_grid(long long n, int threads) {
-  const long long b = (n + threads - 1) / threads;
-  return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+static long long tile_count(long long nx, long long ny) {
+  return ((nx + kTW - 1) / kTW) * ((ny + kTH - 1) / kTH);
+}
+
+static dim3 direct_grid(long long nx, long long ny) {
+  long long bx = (nx + 255) / 256;
+  if (bx > 4096) bx = 4096;
+  return dim3((unsigned)bx, (unsigned)(ny < 65535 ? ny : 65535), 1);
 }
 
 hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t ny,
@@ -171,50 +453,95 @@ hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t
     if (count_host) *count_host = 0;
     return hipSuccess;
   }
-  unsigned long long* count = nullptr;
-  hipError_t e = hipMallocAsync((void**)&count, sizeof(*count), stream);
+  int grid = 1;
+  hipError_t e = tiled_grid((const void*)identifyflats_kernel, 0, tile_count(nx, ny), &grid);
   if (e != hipSuccess) return e;
-  zero_u64<<<1, 1, 0, stream>>>(count);
-  const long long tiles = ((nx + kFlatTW - 1) / kFlatTW) * ((ny + kFlatTH - 1) / kFlatTH);
-  const int grid = (int)(tiles < 16384 ? tiles : 16384);
-  identifyflats_kernel<<<grid, kFlatThreads, 0, stream>>>(out, dem, nx, ny, count);
+  unsigned long long* partial = nullptr;
+  if (count_host) {
+    e = hipMallocAsync((void**)&partial, sizeof(*partial) * (grid + 1), stream);
+    if (e != hipSuccess) return e;
+  }
+  identifyflats_kernel<<<grid, kThreads, 0, stream>>>(out, dem, nx, ny, partial);
   e = hipGetLastError();
-  if (e == hipSuccess && count_host) {
+  if (count_host) {
+    if (e == hipSuccess) {
+      sum_partials<<<1, 256, 0, stream>>>(partial, grid);
+      e = hipGetLastError();
+    }
     unsigned long long h = 0;
-    e = hipMemcpyAsync(&h, count, sizeof(h), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&h, partial + grid, sizeof(h), hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     *count_host = (int64_t)h;
+    hipError_t f = hipFreeAsync(partial, stream);
+    if (e == hipSuccess) e = f;
   }
-  hipError_t f = hipFreeAsync(count, stream);
-  return e != hipSuccess ? e : f;
+  return e;
+}
+
+template <int OP>
+static hipError_t value_filter_launch(float* out, const float* dem, const uint8_t* se,
+                                      const int64_t d[3], int64_t nx, int64_t ny,
+                                      hipStream_t stream) {
+  const bool tiled = d[0] >= 1 && d[1] >= 1 && d[2] >= 1 && d[0] <= 2 * kHaloMax + 1 &&
+                     d[1] <= 2 * kHaloMax + 1 && d[0] * d[1] * d[2] <= kSeMax;
+  if (tiled) {
+    const size_t lds = sizeof(float) * (size_t)(kTW + d[0] - 1) * (size_t)(kTH + d[1] - 1);
+    const int rows = kTH + (int)d[1] - 1;
+    auto go = [&](auto kernel) {
+      int grid = 1;
+      hipError_t e = tiled_grid((const void*)kernel, lds, tile_count(nx, ny), &grid);
+      if (e != hipSuccess) return e;
+      kernel<<<grid, kThreads, lds, stream>>>(out, dem, se, d[0], d[1], d[2], nx, ny);
+      return hipGetLastError();
+    };
+    if (rows <= 5 * kWaves) return go(value_filter_tiled<OP, 5>);
+    if (rows <= 8 * kWaves) return go(value_filter_tiled<OP, 8>);
+    return go(value_filter_tiled<OP, 12>);
+  } else {
+    value_filter_direct<OP><<<direct_grid(nx, ny), 256, 0, stream>>>(out, dem, se, d[0], d[1],
+                                                                     d[2], nx, ny);
+  }
+  return hipGetLastError();
 }
 
 hipError_t run_value_filter(int op, float* out, const float* dem, const uint8_t* se,
                             const int64_t se_dims[3], int64_t nx, int64_t ny,
                             hipStream_t stream) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
-  const int grid = stream_grid(nx * ny, 256);
-  if (op < 0)
-    value_filter_kernel<-1><<<grid, 256, 0, stream>>>(out, dem, se, se_dims[0], se_dims[1],
-                                                      se_dims[2], nx, ny);
-  else
-    value_filter_kernel<1><<<grid, 256, 0, stream>>>(out, dem, se, se_dims[0], se_dims[1],
-                                                     se_dims[2], nx, ny);
+  return op < 0 ? value_filter_launch<-1>(out, dem, se, se_dims, nx, ny, stream)
+                : value_filter_launch<1>(out, dem, se, se_dims, nx, ny, stream);
+}
+
+template <int OP>
+static hipError_t square_launch(float* out, const float* dem, float* tmp, int width, int64_t nx,
+                                int64_t ny, hipStream_t stream) {
+  if (width >= 1 && width <= kSquareMax) {
+    const size_t lds = sizeof(float) * (size_t)(kTH + width - 1) * (size_t)(kTW + width - 1);
+    const int rows = kTH + width - 1;
+    auto go = [&](auto kernel) {
+      int grid = 1;
+      hipError_t e = tiled_grid((const void*)kernel, lds, tile_count(nx, ny), &grid);
+      if (e != hipSuccess) return e;
+      kernel<<<grid, kThreads, lds, stream>>>(out, tmp, dem, width, nx, ny);
+      return hipGetLastError();
+    };
+    if (rows <= 5 * kWaves) return go(value_filter_square_tiled<OP, 5>);
+    if (rows <= 8 * kWaves) return go(value_filter_square_tiled<OP, 8>);
+    return go(value_filter_square_tiled<OP, 12>);
+  } else {
+    const dim3 g = direct_grid(nx, ny);
+    value_filter_axis<OP, 0><<<g, 256, 0, stream>>>(tmp, dem, dem, width, nx, ny);
+    value_filter_axis<OP, 1><<<g, 256, 0, stream>>>(out, tmp, dem, width, nx, ny);
+  }
   return hipGetLastError();
 }
 
 hipError_t run_value_filter_square(int op, float* out, const float* dem, float* tmp, int width,
                                    int64_t nx, int64_t ny, hipStream_t stream) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
-  const int grid = stream_grid(nx * ny, 256);
-  if (op < 0) {
-    value_filter_axis<-1, 0><<<grid, 256, 0, stream>>>(tmp, dem, dem, width, nx, ny);
-    value_filter_axis<-1, 1><<<grid, 256, 0, stream>>>(out, tmp, dem, width, nx, ny);
-  } else {
-    value_filter_axis<1, 0><<<grid, 256, 0, stream>>>(tmp, dem, dem, width, nx, ny);
-    value_filter_axis<1, 1><<<grid, 256, 0, stream>>>(out, tmp, dem, width, nx, ny);
-  }
-  return hipGetLastError();
+  return op < 0 ? square_launch<-1>(out, dem, tmp, width, nx, ny, stream)
+                : square_launch<1>(out, dem, tmp, width, nx, ny, stream);
 }
 
 }  // namespace tt
