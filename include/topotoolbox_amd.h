@@ -104,6 +104,16 @@ void min_filter_square(float *output, float *dem, float *tmp, uint8_t width,
 void max_filter_square(float *output, float *dem, float *tmp, uint8_t width,
                        ptrdiff_t io_dims[2]);
 
+/* Replaces gwdt_computecosts, reference include/topotoolbox.h:464-466
+   (src/gwdt.c:166-274): costs = (largest depth of the flat's 8-connected
+   flat component - its own depth) + 0.1, depth = filled - original;
+   conncomps = the component's label, the same pixel the reference's scan
+   picks (a deepest pixel of the component). Non-flats get 0 and 0. Flats
+   are expected as identifyflats makes them (none on the grid border), with
+   finite depths; nx * ny must be below 2^32. */
+void gwdt_computecosts(float *costs, ptrdiff_t *conncomps, int32_t *flats, float *original_dem,
+                       float *filled_dem, ptrdiff_t dims[2]);
+
 /* ---------------------------------------------------------------------
    Part 2: device-pointer API (no reference counterpart)
    All pointers are device pointers; `stream` is a hipStream_t (NULL = the
@@ -168,6 +178,13 @@ int tt_excesstopography_fsm2d_backward_dev(float *g_dem, float *g_slopes, const 
    stores the number of flats there; with NULL it stays asynchronous. */
 int tt_identifyflats_dev(int32_t *output, const float *dem, ptrdiff_t nx, ptrdiff_t ny,
                          void *stream, int64_t *count);
+
+/* gwdt_computecosts on device buffers. Waits for completion (the number
+   of tie-breaking steps to replay is read back once). Returns nonzero with
+   a message for a NaN depth at a flat or nx * ny >= 2^32. */
+int tt_gwdt_computecosts_dev(float *costs, int64_t *conncomps, const int32_t *flats,
+                             const float *original_dem, const float *filled_dem, ptrdiff_t nx,
+                             ptrdiff_t ny, void *stream);
 
 /* min_filter (op < 0) or max_filter (op > 0) on device buffers;
    `structuring_element` is device memory of se_dims[0]*se_dims[1]*se_dims[2]

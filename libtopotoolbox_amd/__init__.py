@@ -1,7 +1,8 @@
 """libtopotoolbox_amd: MI355X-native sink filling and excess topography.
 
 Drop-in for libtopotoolbox's fillsinks / reconstruct / excesstopography_fsm2d
-hot path and the stencils next to it (identifyflats, min/max filters). The compute runs in HIP kernels for gfx950 behind a C ABI
+hot path and the steps next to it (identifyflats, gwdt_computecosts, min/max
+filters). The compute runs in HIP kernels for gfx950 behind a C ABI
 (include/topotoolbox_amd.h); this package is the host-side mirror of that
 interface for torch tensors (ops), the differentiable wrappers (autograd)
 and the batched tile scheduler (batched).
@@ -14,6 +15,8 @@ from .ops import (
     fillsinks,
     fillsinks_host,
     fillsinks_hybrid_host,
+    gwdt_computecosts,
+    gwdt_computecosts_host,
     identifyflats,
     identifyflats_host,
     max_filter,
@@ -38,6 +41,8 @@ __all__ = [
     "fillsinks",
     "fillsinks_host",
     "fillsinks_hybrid_host",
+    "gwdt_computecosts",
+    "gwdt_computecosts_host",
     "identifyflats",
     "identifyflats_host",
     "max_filter",

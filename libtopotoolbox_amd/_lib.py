@@ -101,6 +101,8 @@ SIGNATURES = {
     "tt_excesstopography_fsm2d_backward_dev": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "gwdt_computecosts": (None, [_vp, _vp, _vp, _vp, _vp, _dims_t]),
+    "tt_gwdt_computecosts_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _vp]),
     "tt_identifyflats_dev": (
         ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(ctypes.c_int64)]),
     "tt_value_filter_dev": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _se_dims_t, _pd, _pd,

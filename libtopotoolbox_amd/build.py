@@ -20,7 +20,7 @@ LIB_NAME = "libtopotoolbox_amd.so"
 LIB_PATH = os.path.join(OUT_DIR, LIB_NAME)
 
 SOURCES = ["runtime.hip", "fill.hip", "fill_graph.hip", "fsm.hip", "adjoint.hip", "batched.hip",
-           "stencil.hip", "capi.hip"]
+           "stencil.hip", "gwdt.hip", "capi.hip"]
 HEADERS = ["tile_engine.h", "kernels.h", "workspace.h"]
 
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")

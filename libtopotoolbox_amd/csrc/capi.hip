@@ -229,6 +229,25 @@ int tt_identifyflats_dev(int32_t* output, const float* dem, ptrdiff_t nx, ptrdif
   return 0;
 }
 
+int tt_gwdt_computecosts_dev(float* costs, int64_t* conncomps, const int32_t* flats,
+                             const float* original_dem, const float* filled_dem, ptrdiff_t nx,
+                             ptrdiff_t ny, void* stream) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || (nx > 0 && (unsigned long long)ny >= 0xffffffffull / (unsigned long long)nx)) {
+    tt::set_error("tt_gwdt_computecosts_dev: dims must be >= 0 with nx * ny < 2^32 - 1");
+    return 1;
+  }
+  unsigned bits = 0;
+  hipError_t e = tt::run_gwdt_computecosts(costs, conncomps, flats, original_dem, filled_dem, nx,
+                                           ny, (hipStream_t)stream, &bits);
+  if (e != hipSuccess) return fail("tt_gwdt_computecosts_dev", e);
+  if (bits) {
+    tt::set_error("tt_gwdt_computecosts_dev: a flat has a NaN depth (filled - original)");
+    return 2;
+  }
+  return 0;
+}
+
 int tt_value_filter_dev(int op, float* output, const float* dem,
                         const uint8_t* structuring_element, const ptrdiff_t se_dims[3],
                         ptrdiff_t nx, ptrdiff_t ny, void* stream) {
@@ -402,6 +421,36 @@ ptrdiff_t identifyflats(int32_t* output, float* dem, ptrdiff_t dims[2]) {
     report("identifyflats");
   }
   return (ptrdiff_t)count;
+}
+
+void gwdt_computecosts(float* costs, ptrdiff_t* conncomps, int32_t* flats, float* original_dem,
+                       float* filled_dem, ptrdiff_t dims[2]) {
+  if (!dims_ok(dims, "gwdt_computecosts")) return report("gwdt_computecosts");
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return;
+  DevBuf d_costs, d_cc, d_flats, d_orig, d_filled;
+  hipError_t e = d_costs.alloc(n * 4);
+  if (e == hipSuccess) e = d_cc.alloc(n * 8);
+  if (e == hipSuccess) e = d_flats.alloc(n * 4);
+  if (e == hipSuccess) e = d_orig.alloc(n * 4);
+  if (e == hipSuccess) e = d_filled.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_flats.p, flats, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_orig.p, original_dem, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_filled.p, filled_dem, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fail("gwdt_computecosts", e);
+    return report("gwdt_computecosts");
+  }
+  if (tt_gwdt_computecosts_dev((float*)d_costs.p, (int64_t*)d_cc.p, (const int32_t*)d_flats.p,
+                               (const float*)d_orig.p, (const float*)d_filled.p, dims[0],
+                               dims[1], nullptr) != 0)
+    return report("gwdt_computecosts");
+  e = hipMemcpy(costs, d_costs.p, n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(conncomps, d_cc.p, n * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fail("gwdt_computecosts", e);
+    report("gwdt_computecosts");
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,397 @@
+// Costs of the gray-weighted distance transform (gwdt_computecosts):
+// connected components of the flats (8-connected) with the largest depth
+// (filled - original) of each, every flat's cost
+// (max depth - own depth) + 0.1 and the component's label.
+//
+// Reference semantics, src/gwdt.c:83-274: one raster scan builds a
+// union-find over the flats, unifying each flat with its flat neighbours
+// among (i-1,j-1), (i,j-1), (i+1,j-1), (i-1,j) in that order; a merge keeps
+// the neighbour's root unless the current pixel's root is strictly deeper.
+// The label is that final root: a pixel of largest depth, and when several
+// pixels share it, the one the scan's merge order picks.
+//
+// Here the components and their largest depth come from a parallel
+// union-find. A component with one pixel of largest depth is labelled with
+// it. Otherwise its label is replayed from the few scan steps that can
+// change which deepest pixel is the root, in scan order, one component per
+// thread:
+//   birth     a flat with no flat scanned neighbour starts a set;
+//   merge     a flat whose scanned neighbours lie in two different sets
+//             (only possible when (i,j-1) is not flat, (i+1,j-1) is, and
+//             (i-1,j-1) or (i-1,j) is); the set met second in the
+//             neighbour order keeps its deepest root, else the first,
+//             else the pixel itself if it is one of the deepest;
+//   first     the first deepest pixel that joins a set grown from one
+//             birth makes it a set with a deepest root, if it has none.
+// All other scan steps leave every deepest root where it is, since only a
+// strictly deeper pixel can displace a root. Sets are tracked by their
+// birth pixel: each flat points at one flat scanned neighbour, and the
+// chains end at births.
+#include <hipcub/hipcub.hpp>
+
+#include "kernels.h"
+
+namespace tt {
+namespace {
+
+constexpr unsigned kNone = 0xffffffffu;
+constexpr unsigned kErrNaN = 1u;
+
+__device__ __forceinline__ unsigned gw_ord(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float gw_unord(unsigned u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ __forceinline__ unsigned ld(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving; every pointer points at a smaller or equal
+// index, so concurrent halving and linking only ever store ancestors.
+__device__ unsigned find_halving(unsigned* P, unsigned x) {
+  for (;;) {
+    const unsigned px = ld(P + x);
+    if (px == x) return x;
+    const unsigned ppx = ld(P + px);
+    if (ppx != px) st(P + x, ppx);
+    x = px;
+  }
+}
+
+// Links the sets of a and b, the larger root under the smaller.
+__device__ void unite(unsigned* P, unsigned a, unsigned b) {
+  for (;;) {
+    a = find_halving(P, a);
+    b = find_halving(P, b);
+    if (a == b) return;
+    if (a < b) {
+      const unsigned t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(P + a, a, b) == a) return;
+  }
+}
+
+struct Grid {
+  const int32_t* flats;
+  long long nx, ny;
+  __device__ bool flat(long long i, long long j) const {
+    return i >= 0 && i < nx && j >= 0 && j < ny && (flats[j * nx + i] & 1);
+  }
+};
+
+// Merge candidate at (i, j): the neighbour of the first set (A: (i-1,j-1)
+// if flat, else (i-1,j)) and whether (i-1,j-1) is flat, which decides the
+// order in which the scan meets the two sets. Returns false if the pixel
+// cannot join two sets.
+__device__ __forceinline__ bool merge_site(const Grid& g, long long i, long long j,
+                                           long long* a_rep, bool* nw) {
+  if (g.flat(i, j - 1) || !g.flat(i + 1, j - 1)) return false;
+  *nw = g.flat(i - 1, j - 1);
+  if (!*nw && !g.flat(i - 1, j)) return false;
+  *a_rep = *nw ? (j - 1) * g.nx + (i - 1) : j * g.nx + (i - 1);
+  return true;
+}
+
+struct Ws {
+  unsigned* A;    // per flat: scanned neighbour, then birth, then component
+  unsigned* C;    // per birth: component union-find (in the costs buffer)
+  unsigned* W;    // per component: largest depth (order-preserving bits)
+  unsigned* Mc;   // per component: number of deepest pixels
+  unsigned* Mf;   // per component: first deepest pixel, then the label
+  unsigned* D;    // per birth: first deepest pixel of its chain group
+  unsigned* F;    // per birth: union-find of the replayed scan (first: roots of A)
+  unsigned* H;    // per replayed set: its deepest root (first: roots of C)
+  unsigned* err;
+};
+
+#define GW_FOR_PIXELS                                                             \
+  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y)                        \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.nx; \
+         i += (long long)gridDim.x * blockDim.x)
+
+__global__ void gw_init(Grid g, Ws w, float* costs, int64_t* conncomps) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1)) {
+      costs[p] = 0.0f;
+      conncomps[p] = 0;
+      continue;
+    }
+    long long par = p;
+    if (g.flat(i - 1, j)) par = p - 1;
+    else if (g.flat(i, j - 1)) par = p - g.nx;
+    else if (g.flat(i - 1, j - 1)) par = p - g.nx - 1;
+    else if (g.flat(i + 1, j - 1)) par = p - g.nx + 1;
+    w.A[p] = (unsigned)par;
+    if (par == p) {
+      w.C[p] = (unsigned)p;
+      w.W[p] = 0u;  // below every ordered float
+      w.Mc[p] = 0u;
+      w.Mf[p] = kNone;
+      w.D[p] = kNone;
+    }
+  }
+}
+
+// Roots are written to a second array and copied back by the next kernel:
+// a root stored into A[p] itself could be overwritten by another thread's
+// path halving that read A[p]'s old parent just before, leaving a
+// non-root behind.
+__global__ void gw_births(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (g.flats[p] & 1) w.F[p] = find_halving(w.A, (unsigned)p);
+  }
+}
+
+__global__ void gw_births_store(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (g.flats[p] & 1) w.A[p] = w.F[p];
+  }
+}
+
+__global__ void gw_components(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    long long a_rep;
+    bool nw;
+    if (!(g.flats[p] & 1) || !merge_site(g, i, j, &a_rep, &nw)) continue;
+    const unsigned a = w.A[a_rep], b = w.A[p - g.nx + 1];
+    if (a != b) unite(w.C, a, b);
+  }
+}
+
+// Component root of every birth, through H and back into C (as above).
+__global__ void gw_component_roots(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if ((g.flats[p] & 1) && w.A[p] == (unsigned)p) w.H[p] = find_halving(w.C, (unsigned)p);
+  }
+}
+
+__global__ void gw_component_store(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if ((g.flats[p] & 1) && w.A[p] == (unsigned)p) w.C[p] = w.H[p];
+  }
+}
+
+__global__ void gw_depth_max(Grid g, Ws w, const float* original, const float* filled) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1)) continue;
+    const float d = filled[p] - original[p];
+    if (d != d) {
+      atomicOr(w.err, kErrNaN);
+      continue;
+    }
+    atomicMax(w.W + w.C[w.A[p]], gw_ord(d));
+  }
+}
+
+__global__ void gw_deepest(Grid g, Ws w, const float* original, const float* filled) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1)) continue;
+    const unsigned b = w.A[p], c = w.C[b];
+    // equal as floats (so -0 and +0 tie, as in the scan's comparison)
+    if (filled[p] - original[p] == gw_unord(w.W[c])) {
+      atomicAdd(w.Mc + c, 1u);
+      atomicMin(w.Mf + c, (unsigned)p);
+      atomicMin(w.D + b, (unsigned)p);
+    }
+  }
+}
+
+// Scan steps to replay (components with two or more deepest pixels), as
+// keys component << 32 | pixel: counted when keys == null, else written.
+__global__ void gw_events(Grid g, Ws w, unsigned long long* keys, unsigned* count) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1)) continue;
+    const unsigned b = w.A[p], c = w.C[b];
+    if (w.Mc[c] < 2u) continue;
+    bool ev = b == (unsigned)p || w.D[b] == (unsigned)p;
+    long long a_rep;
+    bool nw;
+    if (!ev && merge_site(g, i, j, &a_rep, &nw)) ev = w.A[a_rep] != w.A[p - g.nx + 1];
+    if (!ev) continue;
+    const unsigned k = atomicAdd(count, 1u);
+    if (keys) keys[k] = ((unsigned long long)c << 32) | (unsigned long long)p;
+  }
+}
+
+__device__ unsigned find_seq(unsigned* P, unsigned x) {
+  while (P[x] != x) {
+    const unsigned px = P[x];
+    P[x] = P[px];
+    x = px;
+  }
+  return x;
+}
+
+// One thread per component: the replay of its scan steps, in scan order.
+__global__ void gw_replay(Grid g, Ws w, const unsigned long long* keys, unsigned n,
+                          const float* original, const float* filled) {
+  for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const unsigned c = (unsigned)(keys[k] >> 32);
+    if (k > 0 && (unsigned)(keys[k - 1] >> 32) == c) continue;  // not the first of its component
+    const float deepest = gw_unord(w.W[c]);
+    unsigned e = k;
+    for (; e < n && (unsigned)(keys[e] >> 32) == c; e++) {
+      const unsigned p = (unsigned)keys[e];
+      const long long i = (long long)p % g.nx, j = (long long)p / g.nx;
+      const bool pm = filled[p] - original[p] == deepest;
+      const unsigned b = w.A[p];
+      if (b == p) {
+        w.F[p] = p;
+        w.H[p] = pm ? p : kNone;
+        continue;  // a birth is the first deepest pixel of its own group if it is one
+      }
+      long long a_rep;
+      bool nw;
+      if (merge_site(g, i, j, &a_rep, &nw)) {
+        unsigned a = w.A[a_rep], bb = w.A[p - g.nx + 1];
+        if (a != bb) {
+          a = find_seq(w.F, a);
+          bb = find_seq(w.F, bb);
+          if (a != bb) {
+            const unsigned first = nw ? a : bb, second = nw ? bb : a;
+            const unsigned r = w.H[second] != kNone ? w.H[second]
+                               : w.H[first] != kNone ? w.H[first]
+                               : pm                  ? p
+                                                     : kNone;
+            w.F[first] = second;
+            w.H[second] = r;
+          }
+        }
+      }
+      if (w.D[b] == p) {
+        const unsigned s = find_seq(w.F, b);
+        if (w.H[s] == kNone) w.H[s] = p;
+      }
+    }
+    w.Mf[c] = w.H[find_seq(w.F, c)];
+  }
+}
+
+__global__ void gw_to_component(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (g.flats[p] & 1) w.A[p] = w.C[w.A[p]];
+  }
+}
+
+__global__ void gw_final(Grid g, Ws w, const float* original, const float* filled, float* costs,
+                         int64_t* conncomps) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1)) continue;
+    const unsigned c = w.A[p];
+    const float d = filled[p] - original[p];
+    const float gap = gw_unord(w.W[c]) - d;  // float, as the reference
+    costs[p] = (float)((double)gap + 0.1);
+    conncomps[p] = (int64_t)w.Mf[c];
+  }
+}
+
+__global__ void gw_zero(unsigned* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+dim3 pixel_grid(long long nx, long long ny) {
+  long long bx = (nx + 255) / 256;
+  if (bx > 4096) bx = 4096;
+  return dim3((unsigned)bx, (unsigned)(ny < 65535 ? ny : 65535), 1);
+}
+
+}  // namespace
+
+hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t* flats,
+                                 const float* original, const float* filled, int64_t nx,
+                                 int64_t ny, hipStream_t s, unsigned* error_bits) {
+  *error_bits = 0;
+  if (nx <= 0 || ny <= 0) return hipSuccess;
+  const size_t n = (size_t)nx * (size_t)ny;
+  // per-birth union-find C lives in costs (its words are written as costs
+  // only after the last read); the other arrays in one workspace
+  Ws w;
+  unsigned* ws = nullptr;
+  hipError_t e = hipMallocAsync((void**)&ws, (7 * n + 64) * sizeof(unsigned), s);
+  if (e != hipSuccess) return e;
+  w.A = ws;
+  w.W = ws + n;
+  w.Mc = ws + 2 * n;
+  w.Mf = ws + 3 * n;
+  w.D = ws + 4 * n;
+  w.F = ws + 5 * n;
+  w.H = ws + 6 * n;
+  w.err = ws + 7 * n;
+  w.C = reinterpret_cast<unsigned*>(costs);
+  Grid g{flats, nx, ny};
+  const dim3 grid = pixel_grid(nx, ny);
+  unsigned long long* keys = nullptr;
+  unsigned long long* sorted = nullptr;
+  void* temp = nullptr;
+  unsigned nev = 0;
+  auto done = [&](hipError_t r) {
+    if (keys) (void)hipFreeAsync(keys, s);
+    if (sorted) (void)hipFreeAsync(sorted, s);
+    if (temp) (void)hipFreeAsync(temp, s);
+    hipError_t f = hipFreeAsync(ws, s);
+    return r != hipSuccess ? r : f;
+  };
+  gw_zero<<<1, 64, 0, s>>>(w.err, 2);
+  gw_init<<<grid, 256, 0, s>>>(g, w, costs, conncomps);
+  gw_births<<<grid, 256, 0, s>>>(g, w);
+  gw_births_store<<<grid, 256, 0, s>>>(g, w);
+  gw_components<<<grid, 256, 0, s>>>(g, w);
+  gw_component_roots<<<grid, 256, 0, s>>>(g, w);
+  gw_component_store<<<grid, 256, 0, s>>>(g, w);
+  gw_depth_max<<<grid, 256, 0, s>>>(g, w, original, filled);
+  gw_deepest<<<grid, 256, 0, s>>>(g, w, original, filled);
+  unsigned* d_count = w.err + 1;
+  gw_events<<<grid, 256, 0, s>>>(g, w, nullptr, d_count);
+  if ((e = hipGetLastError()) != hipSuccess) return done(e);
+  unsigned host[2] = {0, 0};
+  e = hipMemcpyAsync(host, w.err, sizeof(host), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return done(e);
+  if (host[0]) {
+    *error_bits = host[0];
+    return done(hipSuccess);
+  }
+  nev = host[1];
+  if (nev) {
+    e = hipMallocAsync((void**)&keys, nev * sizeof(unsigned long long), s);
+    if (e == hipSuccess) e = hipMallocAsync((void**)&sorted, nev * sizeof(unsigned long long), s);
+    if (e != hipSuccess) return done(e);
+    gw_zero<<<1, 64, 0, s>>>(d_count, 1);
+    gw_events<<<grid, 256, 0, s>>>(g, w, keys, d_count);
+    size_t temp_bytes = 0;
+    e = hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, keys, sorted, (int)nev, 0, 64, s);
+    if (e == hipSuccess) e = hipMallocAsync(&temp, temp_bytes ? temp_bytes : 1, s);
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortKeys(temp, temp_bytes, keys, sorted, (int)nev, 0, 64, s);
+    if (e != hipSuccess) return done(e);
+    unsigned blocks = (nev + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    gw_replay<<<blocks, 256, 0, s>>>(g, w, sorted, nev, original, filled);
+  }
+  gw_to_component<<<grid, 256, 0, s>>>(g, w);
+  gw_final<<<grid, 256, 0, s>>>(g, w, original, filled, costs, conncomps);
+  return done(hipGetLastError());
+}
+
+}  // namespace tt

@@ -204,6 +204,28 @@ def identifyflats(dem: torch.Tensor, count: bool = True):
     return (out, int(n.value)) if count else out
 
 
+def gwdt_computecosts(flats: torch.Tensor, original_dem: torch.Tensor,
+                      filled_dem: torch.Tensor):
+    """Costs of the gray-weighted distance transform and the flats'
+    connected-component labels (reference gwdt_computecosts,
+    include/topotoolbox.h:407-466): (costs float32, conncomps int64).
+    Synchronises once."""
+    _check_grid(flats, "flats", torch.int32)
+    _check_grid(original_dem, "original_dem")
+    _check_grid(filled_dem, "filled_dem")
+    _same_shape(flats, ("original_dem", original_dem), ("filled_dem", filled_dem))
+    costs = torch.empty(flats.shape, dtype=torch.float32, device=flats.device)
+    cc = torch.empty(flats.shape, dtype=torch.int64, device=flats.device)
+    ny, nx = flats.shape
+    with torch.cuda.device(flats.device):
+        rc = _lib.lib().tt_gwdt_computecosts_dev(costs.data_ptr(), cc.data_ptr(),
+                                                 flats.data_ptr(), original_dem.data_ptr(),
+                                                 filled_dem.data_ptr(), nx, ny,
+                                                 _stream_handle())
+    _lib.check(rc, "gwdt_computecosts")
+    return costs, cc
+
+
 def _se_dims(se) -> tuple:
     """se_dims (sx, sy, slices) of a structuring element shaped (sy, sx) or
     (slices, sy, sx), so that se_dims[0] is the fastest dimension."""
@@ -433,3 +455,21 @@ def min_filter_square_host(dem: np.ndarray, width: int):
 def max_filter_square_host(dem: np.ndarray, width: int):
     """Reference-signature max_filter_square: (output, tmp)."""
     return _value_filter_square_host("max_filter_square", dem, width)
+
+
+def gwdt_computecosts_host(flats: np.ndarray, original_dem: np.ndarray,
+                           filled_dem: np.ndarray):
+    """Reference-signature gwdt_computecosts: (costs, conncomps)."""
+    flats = _host(flats, np.int32)
+    original_dem = _host(original_dem, np.float32)
+    filled_dem = _host(filled_dem, np.float32)
+    costs = np.empty(flats.shape, np.float32)
+    cc = np.empty(flats.shape, np.int64)
+    ny, nx = flats.shape
+    _lib.lib().gwdt_computecosts(costs.ctypes.data, cc.ctypes.data, flats.ctypes.data,
+                                 original_dem.ctypes.data, filled_dem.ctypes.data,
+                                 _lib.dims(nx, ny))
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return costs, cc

@@ -77,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_value_filter_square.restype = None
         L.oracle_value_filter_square.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int, _dims_t,
                                                  ctypes.c_int]
+        L.oracle_gwdt_computecosts.restype = None
+        L.oracle_gwdt_computecosts.argtypes = [_f32p, _i64p, _i32p, _f32p, _f32p, _dims_t]
         L.oracle_fsm2d_adjoint_f64.restype = ctypes.c_int
         L.oracle_fsm2d_adjoint_f64.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p,
                                                ctypes.c_double, _f64p, _dims_t]
@@ -111,6 +113,8 @@ def ref():
         for name in ("min_filter_square", "max_filter_square"):
             getattr(R, name).restype = None
             getattr(R, name).argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint8, _dims_t]
+        R.gwdt_computecosts.restype = None
+        R.gwdt_computecosts.argtypes = [_f32p, _i64p, _i32p, _f32p, _f32p, _dims_t]
         _ref = R
     return _ref
 
@@ -329,3 +333,25 @@ def ref_value_filter_square(dem, width: int, op: str):
     getattr(ref(), f"{op}_filter_square")(out, dem, tmp, int(width), _dims(dem))
     return out
 
+
+
+def gwdt_computecosts(flats, original, filled):
+    """(costs float32, conncomps int64) of the gray-weighted distance
+    transform (src/gwdt.c:166-274)."""
+    flats = _c(flats, np.int32)
+    original = _c(original, np.float32)
+    filled = _c(filled, np.float32)
+    costs = np.empty(flats.shape, np.float32)
+    cc = np.empty(flats.shape, np.int64)
+    lib().oracle_gwdt_computecosts(costs, cc, flats, original, filled, _dims(flats))
+    return costs, cc
+
+
+def ref_gwdt_computecosts(flats, original, filled):
+    flats = _c(flats, np.int32).copy()
+    original = _c(original, np.float32).copy()
+    filled = _c(filled, np.float32).copy()
+    costs = np.empty(flats.shape, np.float32)
+    cc = np.empty(flats.shape, np.int64)
+    ref().gwdt_computecosts(costs, cc, flats, original, filled, _dims(flats))
+    return costs, cc

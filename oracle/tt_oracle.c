@@ -28,6 +28,8 @@
                                  max_filter: op = -1 / +1)
     oracle_value_filter_square   src/value_filters.c:115-246
                                  (min_filter_square, max_filter_square)
+    oracle_gwdt_computecosts     src/gwdt.c:83-274 (union-find over the
+                                 raster scan, heaviest root kept)
 
   Pinning (which check fixes which function) is stated in the tests that
   exercise them: tests/test_oracle.py.
@@ -720,3 +722,88 @@ void oracle_value_filter_square(float *out, const float *dem, float *tmp, int wi
     }
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Costs of the gray-weighted distance transform                      */
+/* ------------------------------------------------------------------ */
+
+/* Union-find stored in (label, weight) arrays: label[p] == p marks a root,
+   whose weight is the largest depth of its set. src/gwdt.c:83-152. */
+static ptrdiff_t gw_root(const ptrdiff_t *label, ptrdiff_t p) {
+  while (label[p] != p) p = label[p];
+  return p;
+}
+
+/* Points every node on the path from p to its root at `root`. */
+static void gw_compress(ptrdiff_t *label, float *weight, ptrdiff_t p, ptrdiff_t root,
+                        float w) {
+  for (;;) {
+    const ptrdiff_t next = label[p];
+    label[p] = root;
+    weight[p] = w;
+    if (next == p) break;
+    p = next;
+  }
+}
+
+/* Joins the sets of a (already scanned neighbour) and p (current pixel):
+   the neighbour's root stays unless p's root is strictly heavier. */
+static void gw_unify(ptrdiff_t *label, float *weight, ptrdiff_t a, ptrdiff_t p) {
+  ptrdiff_t r = gw_root(label, a);
+  float rw = weight[r];
+  if (r != p) {
+    const ptrdiff_t s = gw_root(label, p);
+    if (rw < weight[s]) {
+      r = s;
+      rw = weight[s];
+    }
+    gw_compress(label, weight, p, r, rw);
+  }
+  gw_compress(label, weight, a, r, rw);
+}
+
+/* Cost of every flat: (max depth of its 8-connected flat component - its
+   own depth) + 0.1, the sum taken in double and rounded to float; depth =
+   filled - original in float. conncomps holds the component's root, the
+   pixel whose depth is the largest, chosen by the scan order and tie rule
+   above. Non-flat cells get 0 and 0. src/gwdt.c:166-274. */
+void oracle_gwdt_computecosts(float *costs, ptrdiff_t *conncomps, const int32_t *flats,
+                              const float *original, const float *filled,
+                              const ptrdiff_t dims[2]) {
+  const ptrdiff_t nx = dims[0], ny = dims[1];
+  /* scanned neighbours, in the order the scan unifies them */
+  static const int dj[4] = {-1, -1, -1, 0};
+  static const int di[4] = {-1, 0, 1, -1};
+  for (ptrdiff_t j = 0; j < ny; j++)
+    for (ptrdiff_t i = 0; i < nx; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (!(flats[p] & 1)) {
+        costs[p] = 0.0f;
+        conncomps[p] = 0;
+        continue;
+      }
+      conncomps[p] = p;
+      costs[p] = filled[p] - original[p];
+      for (int k = 0; k < 4; k++) {
+        const ptrdiff_t jj = j + dj[k], ii = i + di[k];
+        if (jj < 0 || jj >= ny || ii < 0 || ii >= nx) continue;
+        const ptrdiff_t q = jj * nx + ii;
+        if (flats[q] & 1) gw_unify(conncomps, costs, q, p);
+      }
+    }
+  for (ptrdiff_t j = 1; j < ny - 1; j++)
+    for (ptrdiff_t i = 1; i < nx - 1; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (!(flats[p] & 1)) continue;
+      const ptrdiff_t r = gw_root(conncomps, p);
+      conncomps[p] = r;
+      costs[p] = costs[r];
+    }
+  for (ptrdiff_t j = 1; j < ny - 1; j++)
+    for (ptrdiff_t i = 1; i < nx - 1; i++) {
+      const ptrdiff_t p = j * nx + i;
+      if (!(flats[p] & 1)) continue;
+      const float depth = filled[p] - original[p];
+      costs[p] = (float)((double)(costs[p] - depth) + 0.1);
+    }
+}

@@ -1,0 +1,124 @@
+"""GPU parity of gwdt_computecosts against the oracle: costs bit for bit
+and conncomps equal (the same deepest pixel as the reference's scan picks,
+including components whose deepest depth is shared by many pixels)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def gwdt_inputs(rng, shape, levels=4, zero_depth=False):
+    flats = (rng.uniform(size=shape) < rng.uniform(0.3, 0.95)).astype(np.int32)
+    flats[0, :] = flats[-1, :] = 0
+    flats[:, 0] = flats[:, -1] = 0
+    original = rng.uniform(0, 10, size=shape).astype(np.float32)
+    depth = rng.integers(0, levels, size=shape).astype(np.float32)
+    if zero_depth:
+        depth[:] = 0
+    return flats, original, (original + depth).astype(np.float32)
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu(flats, orig, filled):
+    import libtopotoolbox_amd as tt
+
+    costs, cc = tt.gwdt_computecosts(cuda(flats), cuda(orig), cuda(filled))
+    return costs.cpu().numpy(), cc.cpu().numpy()
+
+
+def check(flats, orig, filled, oracle_lib, what):
+    want_c, want_cc = oracle_lib.gwdt_computecosts(flats, orig, filled)
+    got_c, got_cc = gpu(flats, orig, filled)
+    np.testing.assert_array_equal(got_cc, want_cc, err_msg=what + " conncomps")
+    assert_bitwise(got_c, want_c, what + " costs")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_ties(oracle_lib, seed):
+    rng = np.random.default_rng(900 + seed)
+    for trial in range(40):
+        shape = tuple(int(v) for v in rng.integers(3, 70, size=2))
+        f, o, fl = gwdt_inputs(rng, shape, int(rng.integers(1, 5)), trial % 4 == 0)
+        check(f, o, fl, oracle_lib, f"seed {seed} trial {trial} {shape}")
+
+
+def test_empty_and_no_flats(oracle_lib):
+    import libtopotoolbox_amd as tt
+
+    for shape in [(1, 1), (2, 5), (6, 1)]:
+        f = np.zeros(shape, np.int32)
+        o = np.zeros(shape, np.float32)
+        check(f, o, o, oracle_lib, f"no flats {shape}")
+    c, cc = tt.gwdt_computecosts(torch.zeros((0, 4), dtype=torch.int32, device="cuda"),
+                                 torch.zeros((0, 4), device="cuda"),
+                                 torch.zeros((0, 4), device="cuda"))
+    assert c.numel() == 0 and cc.numel() == 0
+
+
+def test_filled_dem_pipeline(oracle_lib):
+    """fillsinks -> identifyflats -> gwdt_computecosts on the GPU, against
+    the oracle on the same filled DEM and flats, on quantised DEMs (natural
+    flats of equal depth 0 and filled basins)."""
+    import libtopotoolbox_amd as tt
+
+    rng = np.random.default_rng(44)
+    for k, shape in enumerate([(64, 64), (200, 300), (513, 257)]):
+        dem = np.round(rng.uniform(0, 6, size=shape)).astype(np.float32)
+        if k == 1:
+            dem = np.round(np.cumsum(rng.uniform(-1, 1, size=shape), axis=1) / 2).astype(np.float32)
+        bc = np.zeros(shape, np.uint8)
+        bc[0, :] = bc[-1, :] = 1
+        bc[:, 0] = bc[:, -1] = 1
+        d = cuda(dem)
+        filled = tt.fillsinks(d, cuda(bc))
+        flats, _ = tt.identifyflats(filled)
+        costs, cc = tt.gwdt_computecosts(flats, d, filled)
+        f_np, fl_np = flats.cpu().numpy(), filled.cpu().numpy()
+        want_c, want_cc = oracle_lib.gwdt_computecosts(f_np, dem, fl_np)
+        np.testing.assert_array_equal(cc.cpu().numpy(), want_cc, err_msg=str(shape))
+        assert_bitwise(costs.cpu().numpy(), want_c, str(shape))
+
+
+def test_large_plateaus_and_basins(oracle_lib):
+    """1500 x 2100: one large natural plateau (every pixel of it a deepest
+    one, so the label comes from the replayed scan) next to basins with a
+    unique deepest pixel and basins with tied depths."""
+    rng = np.random.default_rng(7)
+    ny, nx = 1500, 2100
+    flats = np.zeros((ny, nx), np.int32)
+    y, x = np.mgrid[0:ny, 0:nx]
+    plateau = (np.hypot(y - 700, x - 600) < 500 + 80 * np.sin(x / 17.0) * np.cos(y / 23.0))
+    flats[plateau] = 1
+    holes = rng.uniform(size=(ny, nx)) < 0.08
+    flats[holes] = 0
+    basins = (np.hypot(y - 400, x - 1600) < 300) | (np.hypot(y - 1200, x - 1500) < 250)
+    flats[basins] = 1
+    flats[0, :] = flats[-1, :] = 0
+    flats[:, 0] = flats[:, -1] = 0
+    orig = rng.uniform(0, 100, size=(ny, nx)).astype(np.float32)
+    depth = np.zeros((ny, nx), np.float32)
+    depth[basins] = np.round(rng.uniform(0, 3, size=basins.sum())).astype(np.float32)
+    filled = (orig + depth).astype(np.float32)
+    check(flats, orig, filled, oracle_lib, "large")
+
+
+def test_host_dropin_and_nan_error(oracle_lib):
+    import libtopotoolbox_amd as tt
+
+    rng = np.random.default_rng(3)
+    f, o, fl = gwdt_inputs(rng, (50, 60), 2)
+    want_c, want_cc = oracle_lib.gwdt_computecosts(f, o, fl)
+    got_c, got_cc = tt.gwdt_computecosts_host(f, o, fl)
+    np.testing.assert_array_equal(got_cc, want_cc)
+    assert_bitwise(got_c, want_c, "host drop-in")
+    fl2 = fl.copy()
+    idx = tuple(np.argwhere(f == 1)[0])
+    fl2[idx] = np.nan
+    with pytest.raises(Exception, match="NaN depth"):
+        tt.gwdt_computecosts(cuda(f), cuda(o), cuda(fl2))

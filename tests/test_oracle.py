@@ -422,3 +422,53 @@ def test_value_filter_mathematics(oracle_lib):
     for j in range(6):
         for i in range(7):
             assert got[j, i] == small[max(j - 1, 0):j + 2, max(i - 1, 0):i + 2].min()
+
+
+# ------------------------------------------------------------------ gwdt costs
+def gwdt_inputs(rng, shape, levels=4, zero_depth=False):
+    """Random flats (none on the border) with depths from few levels, so
+    that components often hold several deepest pixels."""
+    flats = (rng.uniform(size=shape) < rng.uniform(0.3, 0.95)).astype(np.int32)
+    flats[0, :] = flats[-1, :] = 0
+    flats[:, 0] = flats[:, -1] = 0
+    original = rng.uniform(0, 10, size=shape).astype(np.float32)
+    depth = rng.integers(0, levels, size=shape).astype(np.float32)
+    if zero_depth:
+        depth[:] = 0
+    return flats, original, (original + depth).astype(np.float32)
+
+
+def test_gwdt_computecosts_vs_reference_and_mathematics(oracle_lib):
+    """The oracle against the compiled reference (bitwise), and against what
+    the definition fixes: labels constant on the 8-connected components of
+    the flats (scipy.ndimage.label), each label a deepest pixel of its
+    component, cost = (max depth - depth) + 0.1 rounded from double, zero
+    on non-flats."""
+    from scipy import ndimage
+
+    rng = np.random.default_rng(71)
+    for trial in range(60):
+        shape = tuple(int(v) for v in rng.integers(3, 45, size=2))
+        flats, orig, filled = gwdt_inputs(rng, shape, int(rng.integers(1, 5)), trial % 5 == 0)
+        costs, cc = oracle_lib.gwdt_computecosts(flats, orig, filled)
+        if oracle_lib.ref() is not None:
+            rc, rcc = oracle_lib.ref_gwdt_computecosts(flats, orig, filled)
+            np.testing.assert_array_equal(cc, rcc)
+            np.testing.assert_array_equal(costs.view(np.uint32), rc.view(np.uint32))
+        f = (flats & 1) > 0
+        assert np.all(cc[~f] == 0) and np.all(costs[~f] == 0)
+        lab, n = ndimage.label(f, structure=np.ones((3, 3)))
+        depth = filled - orig
+        for k in range(1, n + 1):
+            m = lab == k
+            labels = np.unique(cc[m])
+            assert labels.size == 1
+            root = int(labels[0])
+            assert m.flat[root]
+            dmax = depth[m].max()
+            assert depth.flat[root] == dmax
+            want = ((dmax - depth[m]).astype(np.float64) + 0.1).astype(np.float32)
+            np.testing.assert_array_equal(costs[m].view(np.uint32), want.view(np.uint32))
+            deepest = np.flatnonzero(m & (depth == dmax))
+            if deepest.size == 1:
+                assert root == deepest[0]
