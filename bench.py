@@ -277,13 +277,17 @@ def batched_run(op: str, count: int, steps: int):
 # Stencils next to the path, bytes per cell: identifyflats reads the DEM and
 # writes int32 flags; a value filter reads the DEM and writes the result; the
 # square filter's contract also writes tmp (the pass along dims[0]).
-STENCIL_BYTES = {"identifyflats": 8, "min_filter_3x3": 8, "max_filter_square_5": 12}
+# gwdt_computecosts reads flats, original and filled DEMs and writes float
+# costs and int64 labels.
+STENCIL_BYTES = {"identifyflats": 8, "min_filter_3x3": 8, "max_filter_square_5": 12,
+                 "gwdt_computecosts": 24}
 
 
 def stencil_run(n: int, steps: int, cpu: bool):
-    """identifyflats, min_filter (3x3) and max_filter_square (width 5) on
-    the filled fill-workload DEM (n x n): launch time from HIP events, the
-    roofline against the bytes above, and the reference's CPU routine on a
+    """identifyflats, min_filter (3x3), max_filter_square (width 5) and
+    gwdt_computecosts (with the flats of the filled DEM) on the filled
+    fill-workload DEM (n x n): time per call from HIP events, the roofline
+    against the bytes above, and the reference's CPU routine on a
     1024 x 1024 crop."""
     import torch
 
@@ -291,8 +295,9 @@ def stencil_run(n: int, steps: int, cpu: bool):
 
     arrays = make_inputs("fill", n)
     dev = torch.device("cuda:0")
-    dem = tt.fillsinks(torch.from_numpy(arrays["dem"]).to(dev),
-                       torch.from_numpy(arrays["bc"]).to(dev))
+    original = torch.from_numpy(arrays["dem"]).to(dev)
+    dem = tt.fillsinks(original, torch.from_numpy(arrays["bc"]).to(dev))
+    flats = tt.identifyflats(dem, count=False)
     del arrays
     se = torch.ones((3, 3), dtype=torch.uint8, device=dev)
     out = torch.empty_like(dem)
@@ -301,6 +306,7 @@ def stencil_run(n: int, steps: int, cpu: bool):
         "identifyflats": lambda: tt.identifyflats(dem, count=False),
         "min_filter_3x3": lambda: tt.min_filter(dem, se, out=out),
         "max_filter_square_5": lambda: tt.max_filter_square(dem, 5, out=out, tmp=tmp),
+        "gwdt_computecosts": lambda: tt.gwdt_computecosts(flats, original, dem),
     }
     res = {}
     crop = None
@@ -308,6 +314,8 @@ def stencil_run(n: int, steps: int, cpu: bool):
         import oracle
 
         crop = np.ascontiguousarray(dem[:1024, :1024].cpu().numpy())
+        crop_orig = np.ascontiguousarray(original[:1024, :1024].cpu().numpy())
+        crop_flats = np.ascontiguousarray(flats[:1024, :1024].cpu().numpy())
     for name, fn in calls.items():
         fn()
         _, per = time_events(fn, steps)
@@ -322,7 +330,9 @@ def stencil_run(n: int, steps: int, cpu: bool):
                    "min_filter_3x3": lambda: oracle.ref_value_filter(
                        crop, np.ones((3, 3), np.uint8), "min"),
                    "max_filter_square_5": lambda: oracle.ref_value_filter_square(
-                       crop, 5, "max")}[name]
+                       crop, 5, "max"),
+                   "gwdt_computecosts": lambda: oracle.ref_gwdt_computecosts(
+                       crop_flats, crop_orig, crop)}[name]
             t0 = time.perf_counter()
             for _ in range(5):
                 ref()
@@ -331,7 +341,7 @@ def stencil_run(n: int, steps: int, cpu: bool):
                                  "cores": 1, "kind": "reference",
                                  "sample": "1024x1024 crop of the filled DEM, 5 calls"}
         res[name] = r
-    del dem, out, tmp
+    del dem, out, tmp, flats, original
     torch.cuda.empty_cache()
     return res
 
@@ -399,6 +409,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stencils", action="store_true")
+    ap.add_argument("--stencils-only", action="store_true",
+                    help="only time the stencils next to the path (identifyflats, value "
+                         "filters, gwdt_computecosts) and print their JSON")
     ap.add_argument("--batch-tiles", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size "
@@ -422,6 +435,10 @@ def main():
     n = args.size
     cells = n * n
     other = "fsm" if args.workload == "fill" else "fill"
+    if args.stencils_only:
+        print(json.dumps({"stencils": stencil_run(n, max(5, args.steps), not args.no_cpu)}),
+              flush=True)
+        return
 
     t0 = time.perf_counter()
     arrays = make_inputs(args.workload, n)

@@ -27,6 +27,13 @@
 // strictly deeper pixel can displace a root. Sets are tracked by their
 // birth pixel: each flat points at one flat scanned neighbour, and the
 // chains end at births.
+//
+// NaN depths (e.g. flats of -INF cells, where filled - original is NaN):
+// the scan's "strictly deeper" test is false whenever a NaN is involved, so
+// a component whose depths are all NaN behaves as one whose depths all tie,
+// and is replayed the same way with every pixel a deepest one. A component
+// mixing NaN and numbers has no such order; its scan is replayed in full,
+// pixel by pixel, with the reference's merge rule.
 #include <hipcub/hipcub.hpp>
 
 #include "kernels.h"
@@ -35,7 +42,8 @@ namespace tt {
 namespace {
 
 constexpr unsigned kNone = 0xffffffffu;
-constexpr unsigned kErrNaN = 1u;
+// component kinds (bits): depths that are numbers, depths that are NaN
+constexpr unsigned kHasNumber = 1u, kHasNaN = 2u, kMixed = 3u;
 
 __device__ __forceinline__ unsigned gw_ord(float f) {
   const unsigned u = __float_as_uint(f);
@@ -103,7 +111,7 @@ __device__ __forceinline__ bool merge_site(const Grid& g, long long i, long long
 }
 
 struct Ws {
-  unsigned* A;    // per flat: scanned neighbour, then birth, then component
+  unsigned* A;    // per flat: run start / chain-group union-find, then birth, then component
   unsigned* C;    // per birth: component union-find (in the costs buffer)
   unsigned* W;    // per component: largest depth (order-preserving bits)
   unsigned* Mc;   // per component: number of deepest pixels
@@ -111,6 +119,7 @@ struct Ws {
   unsigned* D;    // per birth: first deepest pixel of its chain group
   unsigned* F;    // per birth: union-find of the replayed scan (first: roots of A)
   unsigned* H;    // per replayed set: its deepest root (first: roots of C)
+  unsigned* K;    // per component: kind bits (kHasNumber, kHasNaN)
   unsigned* err;
 };
 
@@ -118,6 +127,69 @@ struct Ws {
   for (long long j = blockIdx.y; j < g.ny; j += gridDim.y)                        \
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.nx; \
          i += (long long)gridDim.x * blockDim.x)
+
+// Same pixels, but every lane of a wave runs every iteration (lanes past
+// the row's end with in_row false), for kernels that combine across lanes.
+#define GW_FOR_PIXELS_WAVE                                                         \
+  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y)                         \
+    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < g.nx;             \
+         i0 += (long long)gridDim.x * blockDim.x)                                  \
+      for (long long i = i0 + threadIdx.x, once = 1; once; once = 0)               \
+        if (const bool in_row = i < g.nx; true)
+
+// Combining atomics over a wave: the lanes holding the same key (kNone:
+// no contribution) reduce their values and one of them applies the
+// atomics, once per distinct key in the wave. Must be called by all lanes.
+__device__ void wave_or_max(unsigned* or_to, unsigned* max_to, unsigned key, unsigned orv,
+                            unsigned maxv) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(key != kNone);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const unsigned k0 = __shfl(key, leader);
+    const bool mine = key == k0;
+    unsigned o = mine ? orv : 0u, m = mine ? maxv : 0u;
+    for (int d = 32; d > 0; d >>= 1) {
+      o |= __shfl_xor(o, d);
+      m = max(m, __shfl_xor(m, d));
+    }
+    if (lane == leader) {
+      if (o) atomicOr(or_to + k0, o);
+      if (m) atomicMax(max_to + k0, m);
+    }
+    pending &= ~__ballot(mine);
+  }
+}
+
+__device__ void wave_count_min(unsigned* count_to, unsigned* min_to, unsigned key,
+                               unsigned value) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(key != kNone);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const unsigned k0 = __shfl(key, leader);
+    const unsigned long long group = __ballot(key == k0);
+    unsigned m = key == k0 ? value : kNone;
+    for (int d = 32; d > 0; d >>= 1) m = min(m, __shfl_xor(m, d));
+    if (lane == leader) {
+      if (count_to) atomicAdd(count_to + k0, (unsigned)__popcll(group));
+      atomicMin(min_to + k0, m);
+    }
+    pending &= ~group;
+  }
+}
+
+// One flat scanned neighbour of (i, j), or the pixel itself if it has none
+// (a birth). The trees these links form are the chain groups; their
+// smallest pixel, the root, is their birth.
+__device__ __forceinline__ long long scanned_parent(const Grid& g, long long i, long long j) {
+  const long long p = j * g.nx + i;
+  if (g.flat(i - 1, j)) return p - 1;
+  if (g.flat(i, j - 1)) return p - g.nx;
+  if (g.flat(i - 1, j - 1)) return p - g.nx - 1;
+  if (g.flat(i + 1, j - 1)) return p - g.nx + 1;
+  return p;
+}
 
 __global__ void gw_init(Grid g, Ws w, float* costs, int64_t* conncomps) {
   GW_FOR_PIXELS {
@@ -127,19 +199,59 @@ __global__ void gw_init(Grid g, Ws w, float* costs, int64_t* conncomps) {
       conncomps[p] = 0;
       continue;
     }
-    long long par = p;
-    if (g.flat(i - 1, j)) par = p - 1;
-    else if (g.flat(i, j - 1)) par = p - g.nx;
-    else if (g.flat(i - 1, j - 1)) par = p - g.nx - 1;
-    else if (g.flat(i + 1, j - 1)) par = p - g.nx + 1;
-    w.A[p] = (unsigned)par;
-    if (par == p) {
+    if (scanned_parent(g, i, j) == p) {
       w.C[p] = (unsigned)p;
       w.W[p] = 0u;  // below every ordered float
       w.Mc[p] = 0u;
       w.Mf[p] = kNone;
       w.D[p] = kNone;
+      w.K[p] = 0u;
     }
+  }
+}
+
+// Every flat's scanned parent is its (i-1, j) neighbour when that is flat,
+// so the chain groups are horizontal runs of flats joined through their
+// first pixels. A[p] = the first pixel of p's run: one block per row walks
+// the row in chunks of 256, the last non-flat column carried along.
+__global__ __launch_bounds__(256) void gw_runs(Grid g, Ws w) {
+  __shared__ long long wave_last[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y) {
+    long long carry = -1;  // last non-flat column left of the chunk
+    for (long long i0 = 0; i0 < g.nx; i0 += 256) {
+      const long long i = i0 + threadIdx.x;
+      const bool flat = i < g.nx && (g.flats[j * g.nx + i] & 1);
+      // lanes past the row's end count as non-flat; they follow every pixel
+      const unsigned long long nonflat = __ballot(!flat);
+      const unsigned long long upto =
+          nonflat & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+      const long long base = i0 + 64 * wave;
+      if (lane == 0) wave_last[wave] = nonflat ? base + 63 - __clzll(nonflat) : -1;
+      __syncthreads();
+      long long last = upto ? base + 63 - __clzll(upto) : -1;
+      for (int v = wave - 1; last < 0 && v >= 0; v--) last = wave_last[v];
+      if (last < 0) last = carry;
+      if (flat) w.A[j * g.nx + i] = (unsigned)(j * g.nx + last + 1);
+      for (int v = 3; v >= 0; v--)
+        if (wave_last[v] >= 0) {
+          carry = wave_last[v];
+          break;
+        }
+      __syncthreads();
+    }
+  }
+}
+
+// Joins each run to the run of its first pixel's scanned parent (in the
+// row above); the smaller root always becomes the parent, so each group's
+// root is its smallest pixel, its birth.
+__global__ void gw_link(Grid g, Ws w) {
+  GW_FOR_PIXELS {
+    const long long p = j * g.nx + i;
+    if (!(g.flats[p] & 1) || w.A[p] != (unsigned)p) continue;  // run starts only
+    const long long par = scanned_parent(g, i, j);
+    if (par != p) unite(w.A, (unsigned)p, (unsigned)par);
   }
 }
 
@@ -187,42 +299,56 @@ __global__ void gw_component_store(Grid g, Ws w) {
   }
 }
 
+__device__ __forceinline__ float depth_at(const float* original, const float* filled,
+                                          long long p) {
+  return filled[p] - original[p];
+}
+
 __global__ void gw_depth_max(Grid g, Ws w, const float* original, const float* filled) {
-  GW_FOR_PIXELS {
+  GW_FOR_PIXELS_WAVE {
     const long long p = j * g.nx + i;
-    if (!(g.flats[p] & 1)) continue;
-    const float d = filled[p] - original[p];
-    if (d != d) {
-      atomicOr(w.err, kErrNaN);
-      continue;
+    unsigned c = kNone, kind = 0u, wb = 0u;
+    if (in_row && (g.flats[p] & 1)) {
+      const float d = depth_at(original, filled, p);
+      c = w.C[w.A[p]];
+      kind = d != d ? kHasNaN : kHasNumber;
+      wb = d != d ? 0u : gw_ord(d);
     }
-    atomicMax(w.W + w.C[w.A[p]], gw_ord(d));
+    wave_or_max(w.K, w.W, c, kind, wb);
   }
+}
+
+// Whether p is one of the deepest pixels of its component of kind k.
+__device__ __forceinline__ bool deepest_pixel(unsigned k, float d, unsigned wbits) {
+  return k == kHasNaN ? d != d : d == gw_unord(wbits);  // float ==: -0 and +0 tie
 }
 
 __global__ void gw_deepest(Grid g, Ws w, const float* original, const float* filled) {
-  GW_FOR_PIXELS {
+  GW_FOR_PIXELS_WAVE {
     const long long p = j * g.nx + i;
-    if (!(g.flats[p] & 1)) continue;
-    const unsigned b = w.A[p], c = w.C[b];
-    // equal as floats (so -0 and +0 tie, as in the scan's comparison)
-    if (filled[p] - original[p] == gw_unord(w.W[c])) {
-      atomicAdd(w.Mc + c, 1u);
-      atomicMin(w.Mf + c, (unsigned)p);
-      atomicMin(w.D + b, (unsigned)p);
+    unsigned c = kNone, b = kNone;
+    if (in_row && (g.flats[p] & 1)) {
+      b = w.A[p];
+      const unsigned cc = w.C[b], k = w.K[cc];
+      if (k != kMixed && deepest_pixel(k, depth_at(original, filled, p), w.W[cc])) c = cc;
+      else b = kNone;
     }
+    wave_count_min(w.Mc, w.Mf, c, (unsigned)p);
+    wave_count_min(nullptr, w.D, b, (unsigned)p);
   }
 }
 
-// Scan steps to replay (components with two or more deepest pixels), as
-// keys component << 32 | pixel: counted when keys == null, else written.
+// Scan steps to replay (components with two or more deepest pixels; every
+// pixel of a mixed component), as keys component << 32 | pixel: counted
+// when keys == null, else written.
 __global__ void gw_events(Grid g, Ws w, unsigned long long* keys, unsigned* count) {
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
     if (!(g.flats[p] & 1)) continue;
     const unsigned b = w.A[p], c = w.C[b];
-    if (w.Mc[c] < 2u) continue;
-    bool ev = b == (unsigned)p || w.D[b] == (unsigned)p;
+    const bool mixed = w.K[c] == kMixed;
+    if (!mixed && w.Mc[c] < 2u) continue;
+    bool ev = mixed || b == (unsigned)p || w.D[b] == (unsigned)p;
     long long a_rep;
     bool nw;
     if (!ev && merge_site(g, i, j, &a_rep, &nw)) ev = w.A[a_rep] != w.A[p - g.nx + 1];
@@ -241,18 +367,52 @@ __device__ unsigned find_seq(unsigned* P, unsigned x) {
   return x;
 }
 
+// The reference's scan over every pixel of a mixed component, in scan
+// order: each pixel starts a set and is merged with its flat scanned
+// neighbours in order; a merge keeps the neighbour's root unless the
+// current root is strictly deeper. F is the union-find, H the root (label)
+// of each set.
+__device__ void replay_full(const Grid& g, const Ws& w, const unsigned long long* keys,
+                            unsigned k, unsigned e, const float* original, const float* filled) {
+  const unsigned c = (unsigned)(keys[k] >> 32);
+  for (unsigned q = k; q < e; q++) {
+    const unsigned p = (unsigned)keys[q];
+    const long long i = (long long)p % g.nx, j = (long long)p / g.nx;
+    w.F[p] = p;
+    w.H[p] = p;
+    const long long nb[4][2] = {{i - 1, j - 1}, {i, j - 1}, {i + 1, j - 1}, {i - 1, j}};
+    for (int t = 0; t < 4; t++) {
+      if (!g.flat(nb[t][0], nb[t][1])) continue;
+      const unsigned a = find_seq(w.F, (unsigned)(nb[t][1] * g.nx + nb[t][0]));
+      const unsigned s = find_seq(w.F, p);
+      if (a == s) continue;
+      const unsigned ra = w.H[a], rs = w.H[s];
+      const bool current_deeper = depth_at(original, filled, ra) < depth_at(original, filled, rs);
+      w.F[s] = a;
+      w.H[a] = current_deeper ? rs : ra;
+    }
+  }
+  w.Mf[c] = w.H[find_seq(w.F, c)];
+}
+
 // One thread per component: the replay of its scan steps, in scan order.
 __global__ void gw_replay(Grid g, Ws w, const unsigned long long* keys, unsigned n,
                           const float* original, const float* filled) {
   for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const unsigned c = (unsigned)(keys[k] >> 32);
     if (k > 0 && (unsigned)(keys[k - 1] >> 32) == c) continue;  // not the first of its component
-    const float deepest = gw_unord(w.W[c]);
     unsigned e = k;
-    for (; e < n && (unsigned)(keys[e] >> 32) == c; e++) {
-      const unsigned p = (unsigned)keys[e];
+    while (e < n && (unsigned)(keys[e] >> 32) == c) e++;
+    const unsigned kind = w.K[c];
+    if (kind == kMixed) {
+      replay_full(g, w, keys, k, e, original, filled);
+      continue;
+    }
+    const unsigned wbits = w.W[c];
+    for (unsigned q = k; q < e; q++) {
+      const unsigned p = (unsigned)keys[q];
       const long long i = (long long)p % g.nx, j = (long long)p / g.nx;
-      const bool pm = filled[p] - original[p] == deepest;
+      const bool pm = deepest_pixel(kind, depth_at(original, filled, p), wbits);
       const unsigned b = w.A[p];
       if (b == p) {
         w.F[p] = p;
@@ -298,11 +458,11 @@ __global__ void gw_final(Grid g, Ws w, const float* original, const float* fille
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
     if (!(g.flats[p] & 1)) continue;
-    const unsigned c = w.A[p];
-    const float d = filled[p] - original[p];
-    const float gap = gw_unord(w.W[c]) - d;  // float, as the reference
+    const unsigned label = w.Mf[w.A[p]];
+    // the root's own depth minus this pixel's, in float, as the reference
+    const float gap = depth_at(original, filled, label) - depth_at(original, filled, p);
     costs[p] = (float)((double)gap + 0.1);
-    conncomps[p] = (int64_t)w.Mf[c];
+    conncomps[p] = (int64_t)label;
   }
 }
 
@@ -328,7 +488,7 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   // only after the last read); the other arrays in one workspace
   Ws w;
   unsigned* ws = nullptr;
-  hipError_t e = hipMallocAsync((void**)&ws, (7 * n + 64) * sizeof(unsigned), s);
+  hipError_t e = hipMallocAsync((void**)&ws, (8 * n + 64) * sizeof(unsigned), s);
   if (e != hipSuccess) return e;
   w.A = ws;
   w.W = ws + n;
@@ -337,7 +497,8 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   w.D = ws + 4 * n;
   w.F = ws + 5 * n;
   w.H = ws + 6 * n;
-  w.err = ws + 7 * n;
+  w.K = ws + 7 * n;
+  w.err = ws + 8 * n;
   w.C = reinterpret_cast<unsigned*>(costs);
   Grid g{flats, nx, ny};
   const dim3 grid = pixel_grid(nx, ny);
@@ -354,6 +515,8 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   };
   gw_zero<<<1, 64, 0, s>>>(w.err, 2);
   gw_init<<<grid, 256, 0, s>>>(g, w, costs, conncomps);
+  gw_runs<<<dim3(1, (unsigned)(ny < 65535 ? ny : 65535)), 256, 0, s>>>(g, w);
+  gw_link<<<grid, 256, 0, s>>>(g, w);
   gw_births<<<grid, 256, 0, s>>>(g, w);
   gw_births_store<<<grid, 256, 0, s>>>(g, w);
   gw_components<<<grid, 256, 0, s>>>(g, w);
@@ -368,10 +531,6 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   e = hipMemcpyAsync(host, w.err, sizeof(host), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return done(e);
-  if (host[0]) {
-    *error_bits = host[0];
-    return done(hipSuccess);
-  }
   nev = host[1];
   if (nev) {
     e = hipMallocAsync((void**)&keys, nev * sizeof(unsigned long long), s);

@@ -455,6 +455,17 @@ def test_gwdt_computecosts_vs_reference_and_mathematics(oracle_lib):
             rc, rcc = oracle_lib.ref_gwdt_computecosts(flats, orig, filled)
             np.testing.assert_array_equal(cc, rcc)
             np.testing.assert_array_equal(costs.view(np.uint32), rc.view(np.uint32))
+        if trial % 3 == 1 and oracle_lib.ref() is not None:
+            # NaN depths (INF - INF) scattered among the flats
+            o2, f2 = orig.copy(), filled.copy()
+            m = rng.uniform(size=shape) < 0.2
+            o2[m] = f2[m] = np.inf
+            c2, cc2 = oracle_lib.gwdt_computecosts(flats, o2, f2)
+            rc2, rcc2 = oracle_lib.ref_gwdt_computecosts(flats, o2, f2)
+            np.testing.assert_array_equal(cc2, rcc2)
+            np.testing.assert_array_equal(np.isnan(c2), np.isnan(rc2))
+            ok = ~np.isnan(c2)
+            np.testing.assert_array_equal(c2[ok].view(np.uint32), rc2[ok].view(np.uint32))
         f = (flats & 1) > 0
         assert np.all(cc[~f] == 0) and np.all(costs[~f] == 0)
         lab, n = ndimage.label(f, structure=np.ones((3, 3)))
