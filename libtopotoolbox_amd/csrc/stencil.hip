@@ -98,8 +98,10 @@ struct TileWalk {
 // neighbours). Flat bits are computed for the tile plus 1 cell.
 constexpr int kFlatW = kTW + 4, kFlatH = kTH + 4;
 constexpr int kFlagW = kTW + 2, kFlagH = kTH + 2;
+static_assert(kThreads == kTW, "identifyflats walks one tile column per thread");
+static_assert(2 * kFlagH <= kThreads, "the side flag columns take one thread per cell");
 
-__global__ __launch_bounds__(kThreads) void identifyflats_kernel(
+__global__ __launch_bounds__(kThreads, 5) void identifyflats_kernel(
     int32_t* out, const float* dem, long long nx, long long ny, unsigned long long* partial) {
   __shared__ float sd[kFlatH * kFlatW];
   __shared__ unsigned char sflat[kFlagH * kFlagW];
@@ -119,55 +121,102 @@ __global__ __launch_bounds__(kThreads) void identifyflats_kernel(
     if (tn < w.end)
       win.load(dem, nx, ny, (tn % w.ntx) * kTW - 2, (tn / w.ntx) * kTH - 2, kFlatW, kFlatH);
     // flats: interior cells equal to the minimum of their 3x3 window, NaN
-    // neighbours counting as -INF (a NaN cell is never equal to anything)
-    for (int r = wave; r < kFlagH; r += kWaves) {
-      const long long gy = y0 - 1 + r;
-      const bool row_in = gy > 0 && gy < ny - 1;
-      for (int c = lane; c < kFlagW; c += 64) {
-        const long long gx = x0 - 1 + c;
+    // neighbours counting as -INF (a NaN cell is never equal to anything).
+    // Thread c owns tile column c and walks the flag rows with a 3x3 window
+    // in registers (3 LDS reads per cell); flag column fc is window column
+    // fc + 1, flag row fr window row fr + 1.
+    {
+      const int c = threadIdx.x;  // kThreads == kTW: one column per thread
+      const long long gx = x0 + c;
+      const bool col_in = gx > 0 && gx < nx - 1;
+      float t0 = sd[c + 1], t1 = sd[c + 2], t2 = sd[c + 3];
+      float m0 = sd[kFlatW + c + 1], m1 = sd[kFlatW + c + 2], m2 = sd[kFlatW + c + 3];
+      for (int fr = 0; fr < kFlagH; fr++) {
+        const float* rb = sd + (fr + 2) * kFlatW + c + 1;
+        const float b0 = rb[0], b1 = rb[1], b2 = rb[2];
+        const long long gy = y0 - 1 + fr;
         bool flat = false;
-        if (row_in && gx > 0 && gx < nx - 1) {
-          const float* q = sd + r * kFlatW + c;  // window cell (r + 1, c + 1) is the centre
+        if (col_in && gy > 0 && gy < ny - 1) {
+          float lo = m1;
+          lo = fminf(lo, t0 != t0 ? -__builtin_inff() : t0);
+          lo = fminf(lo, t1 != t1 ? -__builtin_inff() : t1);
+          lo = fminf(lo, t2 != t2 ? -__builtin_inff() : t2);
+          lo = fminf(lo, m0 != m0 ? -__builtin_inff() : m0);
+          lo = fminf(lo, m2 != m2 ? -__builtin_inff() : m2);
+          lo = fminf(lo, b0 != b0 ? -__builtin_inff() : b0);
+          lo = fminf(lo, b1 != b1 ? -__builtin_inff() : b1);
+          lo = fminf(lo, b2 != b2 ? -__builtin_inff() : b2);
+          flat = m1 == lo;
+        }
+        sflat[fr * kFlagW + c + 1] = flat;
+        t0 = m0; t1 = m1; t2 = m2;
+        m0 = b0; m1 = b1; m2 = b2;
+      }
+      // the two flag columns beside the tile (x0 - 1 and x0 + kTW)
+      if (c < 2 * kFlagH) {
+        const int fr = c % kFlagH, fc = c < kFlagH ? 0 : kFlagW - 1;
+        const long long hx = x0 - 1 + fc, hy = y0 - 1 + fr;
+        bool flat = false;
+        if (hx > 0 && hx < nx - 1 && hy > 0 && hy < ny - 1) {
+          const float* q = sd + fr * kFlatW + fc;  // window cell (fr + 1, fc + 1) is the centre
           const float h = q[kFlatW + 1];
-          float lowest = h;
-#pragma unroll
+          float lo = h;
           for (int dr = 0; dr < 3; dr++)
-#pragma unroll
             for (int dc = 0; dc < 3; dc++) {
               if (dr == 1 && dc == 1) continue;
               const float v = q[dr * kFlatW + dc];
-              lowest = fminf(lowest, v != v ? -__builtin_inff() : v);
+              lo = fminf(lo, v != v ? -__builtin_inff() : v);
             }
-          flat = h == lowest;
+          flat = h == lo;
         }
-        sflat[r * kFlagW + c] = flat;
+        sflat[fr * kFlagW + fc] = flat;
       }
     }
     __syncthreads();
-    for (int r = wave; r < kTH; r += kWaves) {
-      const long long gy = y0 + r;
-      if (gy >= ny) break;
-#pragma unroll
-      for (int k = 0; k < kCols; k++) {
-        const int c = lane + 64 * k;
-        const long long gx = x0 + c;
-        if (gx >= nx) continue;
-        const bool flat = sflat[(r + 1) * kFlagW + c + 1];
-        const float h = sd[(r + 2) * kFlatW + c + 2];
-        int32_t bits = flat;
-#pragma unroll
-        for (int dr = -1; dr <= 1; dr++)
-#pragma unroll
-          for (int dc = -1; dc <= 1; dc++) {
-            if (!dr && !dc) continue;
-            // out-of-grid neighbours are NaN in the window and never equal
-            if (sd[(r + 2 + dr) * kFlatW + c + 2 + dc] != h) continue;
-            const bool qflat = sflat[(r + 1 + dr) * kFlagW + c + 1 + dc];
+    // sills and presills, thread c down tile column c with values and flags
+    // of the 3x3 neighbourhood in registers
+    {
+      const int c = threadIdx.x;
+      const long long gx = x0 + c;
+      const bool col_in = gx < nx;
+      const bool has_w = gx > 0, has_e = gx + 1 < nx;
+      // rows r - 1 and r of the window (values) and of the flags
+      float vt0 = sd[1 * kFlatW + c + 1], vt1 = sd[1 * kFlatW + c + 2], vt2 = sd[1 * kFlatW + c + 3];
+      float vm0 = sd[2 * kFlatW + c + 1], vm1 = sd[2 * kFlatW + c + 2], vm2 = sd[2 * kFlatW + c + 3];
+      bool ft0 = sflat[c], ft1 = sflat[c + 1], ft2 = sflat[c + 2];
+      bool fm0 = sflat[kFlagW + c], fm1 = sflat[kFlagW + c + 1], fm2 = sflat[kFlagW + c + 2];
+      for (int r = 0; r < kTH; r++) {
+        const float* vb = sd + (r + 3) * kFlatW + c + 1;
+        const unsigned char* fb = sflat + (r + 2) * kFlagW + c;
+        const float vb0 = vb[0], vb1 = vb[1], vb2 = vb[2];
+        const bool fb0 = fb[0], fb1 = fb[1], fb2 = fb[2];
+        const long long gy = y0 + r;
+        if (col_in && gy < ny) {
+          const bool has_n = gy > 0, has_s = gy + 1 < ny;
+          const float h = vm1;
+          const bool flat = fm1;
+          int32_t bits = flat;
+          // a neighbour outside the grid is NaN in the window: never equal
+          auto look = [&](bool inside, float v, bool qflat) {
+            if (!inside || v != h) return;
             if (!flat && qflat) bits |= 2;  // sill: touches an equal flat
             if (flat && !qflat) bits |= 4;  // presill: a flat touching an equal sill
-          }
-        out[gy * nx + gx] = bits;
-        flats += flat;
+          };
+          look(has_n && has_w, vt0, ft0);
+          look(has_n, vt1, ft1);
+          look(has_n && has_e, vt2, ft2);
+          look(has_w, vm0, fm0);
+          look(has_e, vm2, fm2);
+          look(has_s && has_w, vb0, fb0);
+          look(has_s, vb1, fb1);
+          look(has_s && has_e, vb2, fb2);
+          out[gy * nx + gx] = bits;
+          flats += flat;
+        }
+        vt0 = vm0; vt1 = vm1; vt2 = vm2;
+        vm0 = vb0; vm1 = vb1; vm2 = vb2;
+        ft0 = fm0; ft1 = fm1; ft2 = fm2;
+        fm0 = fb0; fm1 = fb1; fm2 = fb2;
       }
     }
     __syncthreads();  // the next tile's store overwrites sd and sflat
