@@ -4,9 +4,10 @@
 
   Part 1 declares the reference's own entry points with the reference's
   exact signatures (has_topotoolbox, fillsinks, reconstruct,
-  excesstopography_fsm2d, and their hybrid / fast-marching siblings), so a
-  C caller of libtopotoolbox links against libtopotoolbox_amd.so
-  unchanged. They take host pointers, copy to the
+  excesstopography_fsm2d, their hybrid / fast-marching siblings, and the
+  steps next to them: identifyflats, gwdt_computecosts, min_filter,
+  max_filter and their square variants), so a C caller of libtopotoolbox
+  links against libtopotoolbox_amd.so unchanged. They take host pointers, copy to the
   GPU, run the HIP kernels and copy back before returning.
 
   Part 2 adds device-pointer entry points (stream-ordered, for ctypes /
@@ -257,8 +258,8 @@ int tt_batch_run_dev(const void *plan_host, const void *plan_dev,
 
 /* Tile shape used by subsequent calls on this thread, in 64x64 quadrants
    (qx, qy in {1, 2}); returns nonzero for an unsupported shape. Until it is
-   called, fillsinks and reconstruct use 2x2 and excess topography and the
-   spill forward 2x1 (the fastest measured at 8192^2). */
+   called, fillsinks, reconstruct and the spill forward use 2x2 and excess
+   topography 2x1 (the fastest measured at 8192^2). */
 int tt_set_tile_shape(int qx, int qy);
 
 /* Algorithm of fillsinks for subsequent calls on this thread (both give

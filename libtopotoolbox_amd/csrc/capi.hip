@@ -174,7 +174,7 @@ int tt_fillsinks_spill_dev(float* output, int32_t* spill, const float* dem, cons
     return 1;
   }
   tt::RunStats st;
-  hipError_t e = tt::run_fill_spill(output, spill, dem, bc, nx, ny, shape_for({2, 1}),
+  hipError_t e = tt::run_fill_spill(output, spill, dem, bc, nx, ny, shape_for({2, 2}),
                                     (hipStream_t)stream,
                                     stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_fillsinks_spill_dev", e);

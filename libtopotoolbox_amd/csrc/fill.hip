@@ -38,7 +38,9 @@ static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats) {
 template <int MODE>
 static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, RunStats* stats) {
   if constexpr (MODE == kModeSpill) {
-    // 64-bit keys: at most two quadrants fit LDS
+    // 64-bit keys and a kind byte per cell: a 2x2 tile fits LDS alone, a
+    // 2x1 or 1x2 tile twice per CU
+    if (ts.qx == 2 && ts.qy == 2) return launch_fill<TileGeom<2, 2>, MODE>(a, stream, stats);
     if (ts.qx == 1 && ts.qy == 1) return launch_fill<TileGeom<1, 1>, MODE>(a, stream, stats);
     if (ts.qx == 1 && ts.qy == 2) return launch_fill<TileGeom<1, 2>, MODE>(a, stream, stats);
     return launch_fill<TileGeom<2, 1>, MODE>(a, stream, stats);

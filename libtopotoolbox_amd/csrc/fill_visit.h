@@ -45,9 +45,10 @@ struct FillView {
   const float* src;
   const uint8_t* bc;
   long long nx, ny;
-  // kModeSpill: state already holds the exact fill levels F, so a cell
-  // starts at (F, no index) instead of (+INF, no index): a tighter upper
-  // bound of its key, which leaves only the index part to settle
+  // kModeSpill: must be true: state holds the exact fill levels F, so a
+  // cell starts at (F, no index) instead of (+INF, no index): a tighter
+  // upper bound of its key, which leaves only the index part to settle, and
+  // each cell's bound reduces to a kind byte (SpillBound)
   bool levels;
 };
 
@@ -184,11 +185,141 @@ __device__ __forceinline__ bool fill_sweep(typename L::T* sf, const typename L::
   return changed;
 }
 
+// kModeSpill bound of a cell from its kind byte. With the levels known, a
+// cell's bound key (ord(d), own index) matters only where d equals its
+// level (kSurface: the bound is (own level, own index)); a submerged cell's
+// bound lies below every key its neighbours can offer (their levels are
+// at least its own), and NaN cells are inert.
+enum : unsigned char { kSubmerged = 0, kSurface = 1, kInertCell = 2 };
+
+struct SpillBound {
+  const unsigned char* kind;  // LDS, one byte per cell (TileGeom layout)
+  unsigned base;              // grid index of tile cell (0, 0)
+  unsigned nx;
+  // bound of the cell at LDS index `at`, tile row r, column c, whose key is `own`
+  __device__ __forceinline__ unsigned long long operator()(int at, int r, int c,
+                                                           unsigned long long own) const {
+    const unsigned k = kind[at];
+    if (k == kSubmerged) return 0ull;
+    if (k == kInertCell) return KeyLevel::kInert;
+    return (own & 0xffffffff00000000ull) | (unsigned)(base + (unsigned)r * nx + (unsigned)c);
+  }
+};
+
+// Surface bits of a lane's 64 cells in the column layout (bit i: cell
+// (R0 + i, C0 + lane)) or the row layout (bit i: cell (R0 + lane, C0 + i)),
+// taken once per visit from the kind bytes, which do not change during it.
+template <class G, bool VERT>
+__device__ __forceinline__ unsigned long long surface_bits(const unsigned char* kind, int R0,
+                                                           int C0, int lane) {
+  unsigned long long m = 0ull;
+#pragma unroll 8
+  for (int i = 0; i < 64; i++) {
+    const int at = VERT ? G::at(R0 + i, C0 + lane) : G::at(R0 + lane, C0 + i);
+    m |= (unsigned long long)(kind[at] == kSurface) << i;
+  }
+  return m;
+}
+
+// One directional spill sweep (as the order-domain sweeps, over 64-bit
+// keys in LDS). The bound of the lane's cell i of its line: inert cells
+// hold the inert key (all ones) themselves; otherwise (own level, own
+// index) on surface cells, nothing (0) on submerged ones.
+template <class G, bool VERT, bool FWD>
+__device__ __forceinline__ bool spill_sweep(unsigned long long* sf, unsigned long long surf,
+                                            const SpillBound& bound, int R0, int C0, int lane) {
+  using L = KeyLevel;
+  using T = typename L::T;
+  constexpr int NCH = kChains, LEN = 64 / kChains;
+  constexpr int step = FWD ? 1 : -1;
+  bool changed = false;
+  const int own = VERT ? C0 + lane : R0 + lane;      // lane's column / row
+  const int side = (lane == 0) ? own - 1 : own + 1;  // outside neighbour line
+  const int base = VERT ? R0 : C0;                   // line position 0
+  const int k0 = FWD ? base : base + 63;
+  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
+  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
+  // grid index of line position i: line0 + i * stride
+  const unsigned stride = VERT ? bound.nx : 1u;
+  const unsigned line0 = VERT ? bound.base + (unsigned)R0 * bound.nx + (unsigned)own
+                              : bound.base + (unsigned)own * bound.nx + (unsigned)C0;
+  T pc[NCH];
+  T fw[NCH][kPrefetch], ew[NCH][kPrefetch];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int kc = k0 + c * LEN * step;
+    pc[c] = sf[cell(kc - step)];
+#pragma unroll
+    for (int i = 0; i < kPrefetch; i++) {
+      fw[c][i] = sf[cell(kc + i * step)];
+      ew[c][i] = sf[edge_cell(kc + (i - 1) * step)];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < LEN; s++) {
+    const int slot = s % kPrefetch;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const int li = FWD ? c * LEN + s : 63 - (c * LEN + s);  // line position (static)
+      const int k = base + li;
+      const T fc = fw[c][slot], edge = ew[c][slot];
+      if (s + kPrefetch < LEN) {
+        fw[c][slot] = sf[cell(k + kPrefetch * step)];
+        ew[c][slot] = sf[edge_cell(k + (kPrefetch - 1) * step)];
+      }
+      const T dv = fc == L::kInert             ? L::kInert
+                   : ((surf >> li) & 1ull) != 0 ? ((fc & 0xffffffff00000000ull) |
+                                                  (line0 + (unsigned)li * stride))
+                                                : 0ull;
+      const T a = L::from_below(pc[c], edge);
+      const T b = L::from_above(pc[c], edge);
+      const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc[c], fc)));
+      changed |= L::less(nv, fc);
+      pc[c] = L::lo(nv, fc);
+      sf[cell(k)] = pc[c];
+    }
+  }
+  return changed;
+}
+
 // Fixed-point test of a wavefront's quadrant: true (in some lane) if a
 // cell would still decrease. Lanes are columns walking the rows with a
 // three-row window in registers; the side columns come by DPP (the
 // quadrant's outside columns from LDS for the edge lanes). The steps are
 // independent, so this costs a fraction of a sweep.
+template <class G>
+__device__ __forceinline__ bool spill_violation(const unsigned long long* sf,
+                                                unsigned long long vsurf, const SpillBound& bound,
+                                                int R0, int C0, int lane) {
+  using L = KeyLevel;
+  using T = typename L::T;
+  const int C = C0 + lane;
+  const int S = (lane == 0) ? C - 1 : C + 1;  // outside column for the edge lanes
+  const unsigned line0 = bound.base + (unsigned)R0 * bound.nx + (unsigned)C;
+  T up = sf[G::at(R0 - 1, C)], mid = sf[G::at(R0, C)];
+  T eup = sf[G::at(R0 - 1, S)], emid = sf[G::at(R0, S)];
+  bool v = false;
+#pragma unroll 8
+  for (int r = 0; r < 64; r++) {
+    const T dn = sf[G::at(R0 + r + 1, C)];
+    const T edn = sf[G::at(R0 + r + 1, S)];
+    const T dv = mid == L::kInert             ? L::kInert
+                 : ((vsurf >> r) & 1ull) != 0 ? ((mid & 0xffffffff00000000ull) |
+                                                (line0 + (unsigned)r * bound.nx))
+                                              : 0ull;
+    const T col = L::lo(L::lo(up, dn), mid);       // own column, rows r-1..r+1
+    const T ecol = L::lo(L::lo(eup, edn), emid);   // outside column (edge lanes)
+    const T m = L::lo(col, L::lo(L::from_below(col, ecol), L::from_above(col, ecol)));
+    v |= L::less(L::hi(dv, m), mid);
+    up = mid;
+    mid = dn;
+    eup = emid;
+    emid = edn;
+  }
+  return v;
+}
+
+// The same test for the fill / reconstruction order domain (bounds in sd).
 template <class G, class L>
 __device__ __forceinline__ bool fill_violation(const typename L::T* sf, const typename L::T* sd,
                                                int R0, int C0, int lane) {
@@ -203,8 +334,8 @@ __device__ __forceinline__ bool fill_violation(const typename L::T* sf, const ty
     const T dn = sf[G::at(R0 + r + 1, C)];
     const T edn = sf[G::at(R0 + r + 1, S)];
     const T dv = sd[G::at(R0 + r, C)];
-    const T col = L::lo(L::lo(up, dn), mid);       // own column, rows r-1..r+1
-    const T ecol = L::lo(L::lo(eup, edn), emid);   // outside column (edge lanes)
+    const T col = L::lo(L::lo(up, dn), mid);
+    const T ecol = L::lo(L::lo(eup, edn), emid);
     const T m = L::lo(col, L::lo(L::from_below(col, ecol), L::from_above(col, ecol)));
     v |= L::less(L::hi(dv, m), mid);
     up = mid;
@@ -306,11 +437,14 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
 }
 
 // LDS of one fill visit.
+// kModeSpill keeps one kind byte per cell instead of a bound key (see
+// SpillBound), so that a 128x64 tile fits two workgroups per CU.
 template <class G, int MODE>
 struct FillLds {
   using T = typename FillLevel<MODE>::T;
+  using B = typename std::conditional<MODE == kModeSpill, unsigned char, T>::type;
   T sf[G::CELLS];
-  T sd[G::CELLS];
+  B sd[G::CELLS];
   T sper[2 * (G::TX + G::TY)];
   unsigned bits;
 };
@@ -326,7 +460,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   using L = FillLevel<MODE>;
   using T = typename L::T;
   T* sf = lds.sf;
-  T* sd = lds.sd;
+  auto* sd = lds.sd;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -338,27 +472,30 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
     constexpr int E = G::ROWS * (G::TX + 2);
     constexpr int U = 8;
     for (int base = 0; base < E; base += G::THREADS * U) {
-      T fv[U], dv[U];
+      T fv[U];
+      unsigned char kv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int idx = base + u * G::THREADS + tid;
         const int r = idx / (G::TX + 2) - 1;
         const int c = idx % (G::TX + 2) - 1;
         const long long gx = x0 + c, gy = y0 + r;
-        T f = L::kUnreached, d = L::kInert;
+        T f = L::kInert;  // NaN and outside cells: never lowered, never lower
+        unsigned char kind = kInertCell;
         if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
           const size_t g = (size_t)gy * v.nx + gx;
           const float raw = v.src[g];
           if (raw == raw) {
-            d = KeyLevel::make(raw, (unsigned)g);
-            f = first ? ((v.bc[g] == 1) ? d
-                                        : (v.levels ? KeyLevel::make(v.state[g], 0xffffffffu)
-                                                    : KeyLevel::kUnreached))
+            const T d = KeyLevel::make(raw, (unsigned)g);
+            // every key's level half is the cell's fill level (the levels
+            // were staged as the start keys on the first visit)
+            f = first ? ((v.bc[g] == 1) ? d : KeyLevel::make(v.state[g], 0xffffffffu))
                       : v.keys[g];
+            kind = (unsigned)(f >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
           }
         }
         fv[u] = f;
-        dv[u] = d;
+        kv[u] = kind;
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -367,7 +504,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
           const int r = idx / (G::TX + 2) - 1;
           const int c = idx % (G::TX + 2) - 1;
           sf[G::at(r, c)] = fv[u];
-          sd[G::at(r, c)] = dv[u];
+          sd[G::at(r, c)] = kv[u];
         }
       }
     }
@@ -410,11 +547,13 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       const float raw = v.src[g];
       if (raw != raw) return;  // inert either way
       if constexpr (MODE == kModeSpill) {
-        sf[G::at(r, c)] =
-            from_state ? v.keys[g]
-                       : ((v.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
-                                         : (v.levels ? KeyLevel::make(v.state[g], 0xffffffffu)
-                                                     : KeyLevel::kUnreached));
+        const T k = from_state ? v.keys[g]
+                               : ((v.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
+                                                 : KeyLevel::make(v.state[g], 0xffffffffu));
+        sf[G::at(r, c)] = k;
+        // the staged word (and the kind taken from its level) may have been
+        // an unvisited neighbour's uninitialised key
+        sd[G::at(r, c)] = (unsigned)(k >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
       } else {
         sf[G::at(r, c)] = ord_bits(from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf));
       }
@@ -428,15 +567,19 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // spill keys from known levels, first visit: an unfilled cell (level ==
   // its own d) with a strictly lower neighbour keeps (d, own index) at the
   // fixed point; seeding it there leaves only plateau cells to settle
+  // the bound of each cell (kModeSpill); tile cell (0, 0) is grid cell
+  // y0 * nx + x0, below 2^31 like every spill index
+  const SpillBound bound{reinterpret_cast<const unsigned char*>(lds.sd),
+                         (unsigned)(y0 * v.nx + x0), (unsigned)v.nx};
   if constexpr (MODE == kModeSpill) {
-    if (first && v.levels) {
+    if (first) {
       __syncthreads();  // perimeter saved before any cell is seeded
       for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
         const int r = idx / G::TX, c = idx % G::TX;
-        const T k = sf[G::at(r, c)], d = sd[G::at(r, c)];
-        if ((unsigned)k != 0xffffffffu || d == KeyLevel::kInert) continue;
+        const T k = sf[G::at(r, c)];
+        // unreached index, at its own level (filled cells lie above d)
+        if ((unsigned)k != 0xffffffffu || sd[G::at(r, c)] != kSurface) continue;
         const unsigned lev = (unsigned)(k >> 32);
-        if (lev != (unsigned)(d >> 32)) continue;  // filled: level above d
         unsigned lowest = 0xffffffffu;
 #pragma unroll
         for (int dr = -1; dr <= 1; dr++)
@@ -445,7 +588,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
             const unsigned l = (unsigned)(sf[G::at(r + dr, c + dc)] >> 32);
             lowest = l < lowest ? l : lowest;
           }
-        if (lowest < lev) sf[G::at(r, c)] = d;
+        if (lowest < lev) sf[G::at(r, c)] = bound(G::at(r, c), r, c, k);
       }
       __syncthreads();
     }
@@ -456,16 +599,19 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   unsigned it = 0;
   int any = 0, any_change = 0;
   if constexpr (MODE == kModeSpill) {
-    any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+    const unsigned long long vsurf = surface_bits<G, true>(bound.kind, R0, C0, lane);
+    const unsigned long long hsurf = surface_bits<G, false>(bound.kind, R0, C0, lane);
+    any = __syncthreads_or(spill_violation<G>(sf, vsurf, bound, R0, C0, lane) ? 1 : 0);
     while (any && it < max_local) {
-      bool ch = fill_sweep<G, L, true, true>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, L, true, false>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, L, false, true>(sf, sd, R0, C0, lane);
-      ch |= fill_sweep<G, L, false, false>(sf, sd, R0, C0, lane);
+      bool ch = spill_sweep<G, true, true>(sf, vsurf, bound, R0, C0, lane);
+      ch |= spill_sweep<G, true, false>(sf, vsurf, bound, R0, C0, lane);
+      ch |= spill_sweep<G, false, true>(sf, hsurf, bound, R0, C0, lane);
+      ch |= spill_sweep<G, false, false>(sf, hsurf, bound, R0, C0, lane);
       any = __syncthreads_or(ch ? 1 : 0);
       any_change |= any;
       it++;
-      if (any) any = __syncthreads_or(fill_violation<G, L>(sf, sd, R0, C0, lane) ? 1 : 0);
+      if (any)
+        any = __syncthreads_or(spill_violation<G>(sf, vsurf, bound, R0, C0, lane) ? 1 : 0);
     }
   } else {
     it = ord_relax_regs<G>(sf, sd, R0, C0, lane, max_local, &any, &any_change);
@@ -481,7 +627,11 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       if (gx < v.nx && gy < v.ny) {
         const size_t g = (size_t)gy * v.nx + gx;
         const T val = sf[G::at(r, c)];
-        const bool inert = sd[G::at(r, c)] == L::kInert;
+        bool inert;
+        if constexpr (MODE == kModeSpill)
+          inert = sd[G::at(r, c)] == kInertCell;
+        else
+          inert = sd[G::at(r, c)] == L::kInert;
         if constexpr (MODE == kModeSpill) {
           // one aligned 64-bit store: a neighbour reading its halo
           // concurrently sees the old or the new key, never a mix
@@ -502,7 +652,13 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // ---- changed perimeter -> neighbour tiles whose cells it may lower
   unsigned bits = block_or(pushes_from_perimeter<G, true>(lds.sper, sf,
                                                           [&](int p, int q) {
-                                                            return L::less(L::hi(sd[q], sf[p]),
+                                                            T dq;
+                                                            if constexpr (MODE == kModeSpill)
+                                                              dq = bound(q, q / G::P - 1,
+                                                                         q % G::P - 1, sf[q]);
+                                                            else
+                                                              dq = sd[q];
+                                                            return L::less(L::hi(dq, sf[p]),
                                                                            sf[q]);
                                                           }),
                            &lds.bits);
