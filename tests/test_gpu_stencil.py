@@ -143,3 +143,38 @@ def test_stencils_large(oracle_lib):
                    oracle_lib.value_filter(dem, se, "max"), "max cross")
     assert_bitwise(tt.min_filter_square(d, 7).cpu().numpy(),
                    oracle_lib.value_filter_square(dem, 7, "min"), "min square 7")
+
+
+@pytest.mark.slow
+def test_stencils_and_costs_8192(oracle_lib):
+    """The 8192^2 fill workload (fractal with depressions and a -INF blob of
+    bc cells): fill on the GPU, then identifyflats, min_filter 3x3,
+    max_filter_square 5 and gwdt_computecosts on the filled DEM, each
+    against the oracle on the same arrays (bitwise; NaN costs, from INF -
+    INF depths in the blob, as NaN)."""
+    import libtopotoolbox_amd as tt
+    from libtopotoolbox_amd import synthetic as S
+
+    n = 8192
+    dem = S.dem_depressions(n, n, 2)
+    dem, bc, _ = S.nan_to_bc(dem, S.edge_bc(n, n))
+    d = cuda(dem)
+    filled = tt.fillsinks(d, cuda(bc))
+    flags, nflats = tt.identifyflats(filled)
+    f_np = filled.cpu().numpy()
+    want, wn = oracle_lib.identifyflats(f_np)
+    np.testing.assert_array_equal(flags.cpu().numpy(), want)
+    assert nflats == wn
+    se = np.ones((3, 3), np.uint8)
+    assert_bitwise(tt.min_filter(filled, cuda(se)).cpu().numpy(),
+                   oracle_lib.value_filter(f_np, se, "min"), "min_filter 3x3")
+    assert_bitwise(tt.max_filter_square(filled, 5).cpu().numpy(),
+                   oracle_lib.value_filter_square(f_np, 5, "max"), "max_filter_square 5")
+    costs, cc = tt.gwdt_computecosts(flags, d, filled)
+    want_c, want_cc = oracle_lib.gwdt_computecosts(want, dem, f_np)
+    np.testing.assert_array_equal(cc.cpu().numpy(), want_cc)
+    got_c = costs.cpu().numpy()
+    nan = np.isnan(want_c)
+    np.testing.assert_array_equal(np.isnan(got_c), nan)
+    assert_bitwise(got_c[~nan], want_c[~nan], "gwdt costs")
+    print(f"8192^2: {nflats} flats; stencils and costs bit-exact vs the oracle")
