@@ -77,6 +77,10 @@ SES = {
                         np.array([[1, 0, 1], [0, 0, 0], [1, 0, 1]], np.uint8)]),
     "empty": np.zeros((3, 3), np.uint8),
     "9x9": np.ones((9, 9), np.uint8),
+    # past the LDS path's limits (33 cells per axis, 1024 entries): the
+    # direct kernel
+    "35x1": np.ones((1, 35), np.uint8),
+    "3x3x120": (np.arange(1080).reshape(120, 3, 3) % 7 != 3).astype(np.uint8),
 }
 
 
