@@ -111,8 +111,8 @@ __device__ __forceinline__ bool merge_site(const Grid& g, long long i, long long
 }
 
 struct Ws {
-  unsigned* A;    // per flat: run start / chain-group union-find, then birth, then component
-  unsigned* C;    // per birth: component union-find (in the costs buffer)
+  unsigned* A;    // per flat: run start / chain-group union-find, then birth
+  unsigned* C;    // per birth: component union-find, then component
   unsigned* W;    // per component: largest depth (order-preserving bits)
   unsigned* Mc;   // per component: number of deepest pixels
   unsigned* Mf;   // per component: first deepest pixel, then the label
@@ -191,29 +191,12 @@ __device__ __forceinline__ long long scanned_parent(const Grid& g, long long i, 
   return p;
 }
 
-__global__ void gw_init(Grid g, Ws w, float* costs, int64_t* conncomps) {
-  GW_FOR_PIXELS {
-    const long long p = j * g.nx + i;
-    if (!(g.flats[p] & 1)) {
-      costs[p] = 0.0f;
-      conncomps[p] = 0;
-      continue;
-    }
-    if (scanned_parent(g, i, j) == p) {
-      w.C[p] = (unsigned)p;
-      w.W[p] = 0u;  // below every ordered float
-      w.Mc[p] = 0u;
-      w.Mf[p] = kNone;
-      w.D[p] = kNone;
-      w.K[p] = 0u;
-    }
-  }
-}
-
 // Every flat's scanned parent is its (i-1, j) neighbour when that is flat,
 // so the chain groups are horizontal runs of flats joined through their
 // first pixels. A[p] = the first pixel of p's run: one block per row walks
-// the row in chunks of 256, the last non-flat column carried along.
+// the row in chunks of 256, the last non-flat column carried along. The
+// births (run starts with no flat in the row above) initialise their
+// per-birth entries.
 __global__ __launch_bounds__(256) void gw_runs(Grid g, Ws w) {
   __shared__ long long wave_last[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -232,7 +215,19 @@ __global__ __launch_bounds__(256) void gw_runs(Grid g, Ws w) {
       long long last = upto ? base + 63 - __clzll(upto) : -1;
       for (int v = wave - 1; last < 0 && v >= 0; v--) last = wave_last[v];
       if (last < 0) last = carry;
-      if (flat) w.A[j * g.nx + i] = (unsigned)(j * g.nx + last + 1);
+      if (flat) {
+        const long long p = j * g.nx + i;
+        w.A[p] = (unsigned)(j * g.nx + last + 1);
+        if (last == i - 1 && !g.flat(i - 1, j - 1) && !g.flat(i, j - 1) &&
+            !g.flat(i + 1, j - 1)) {  // a birth
+          w.C[p] = (unsigned)p;
+          w.W[p] = 0u;  // below every ordered float
+          w.Mc[p] = 0u;
+          w.Mf[p] = kNone;
+          w.D[p] = kNone;
+          w.K[p] = 0u;
+        }
+      }
       for (int v = 3; v >= 0; v--)
         if (wave_last[v] >= 0) {
           carry = wave_last[v];
@@ -446,20 +441,19 @@ __global__ void gw_replay(Grid g, Ws w, const unsigned long long* keys, unsigned
   }
 }
 
-__global__ void gw_to_component(Grid g, Ws w) {
-  GW_FOR_PIXELS {
-    const long long p = j * g.nx + i;
-    if (g.flats[p] & 1) w.A[p] = w.C[w.A[p]];
-  }
-}
-
+// Every pixel's outputs: 0 and 0 for non-flats; for flats the component's
+// label and (root depth - own depth, in float) + 0.1 rounded from double,
+// as the reference.
 __global__ void gw_final(Grid g, Ws w, const float* original, const float* filled, float* costs,
                          int64_t* conncomps) {
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
-    if (!(g.flats[p] & 1)) continue;
-    const unsigned label = w.Mf[w.A[p]];
-    // the root's own depth minus this pixel's, in float, as the reference
+    if (!(g.flats[p] & 1)) {
+      costs[p] = 0.0f;
+      conncomps[p] = 0;
+      continue;
+    }
+    const unsigned label = w.Mf[w.C[w.A[p]]];
     const float gap = depth_at(original, filled, label) - depth_at(original, filled, p);
     costs[p] = (float)((double)gap + 0.1);
     conncomps[p] = (int64_t)label;
@@ -484,11 +478,11 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   *error_bits = 0;
   if (nx <= 0 || ny <= 0) return hipSuccess;
   const size_t n = (size_t)nx * (size_t)ny;
-  // per-birth union-find C lives in costs (its words are written as costs
-  // only after the last read); the other arrays in one workspace
+  // per-pixel arrays in one workspace (the outputs are written only by the
+  // last kernel)
   Ws w;
   unsigned* ws = nullptr;
-  hipError_t e = hipMallocAsync((void**)&ws, (8 * n + 64) * sizeof(unsigned), s);
+  hipError_t e = hipMallocAsync((void**)&ws, (9 * n + 64) * sizeof(unsigned), s);
   if (e != hipSuccess) return e;
   w.A = ws;
   w.W = ws + n;
@@ -498,8 +492,8 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   w.F = ws + 5 * n;
   w.H = ws + 6 * n;
   w.K = ws + 7 * n;
-  w.err = ws + 8 * n;
-  w.C = reinterpret_cast<unsigned*>(costs);
+  w.C = ws + 8 * n;
+  w.err = ws + 9 * n;  // [0] unused, [1] replay count
   Grid g{flats, nx, ny};
   const dim3 grid = pixel_grid(nx, ny);
   unsigned long long* keys = nullptr;
@@ -514,7 +508,6 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
     return r != hipSuccess ? r : f;
   };
   gw_zero<<<1, 64, 0, s>>>(w.err, 2);
-  gw_init<<<grid, 256, 0, s>>>(g, w, costs, conncomps);
   gw_runs<<<dim3(1, (unsigned)(ny < 65535 ? ny : 65535)), 256, 0, s>>>(g, w);
   gw_link<<<grid, 256, 0, s>>>(g, w);
   gw_births<<<grid, 256, 0, s>>>(g, w);
@@ -548,7 +541,6 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
     if (blocks > 16384) blocks = 16384;
     gw_replay<<<blocks, 256, 0, s>>>(g, w, sorted, nev, original, filled);
   }
-  gw_to_component<<<grid, 256, 0, s>>>(g, w);
   gw_final<<<grid, 256, 0, s>>>(g, w, original, filled, costs, conncomps);
   return done(hipGetLastError());
 }
