@@ -93,7 +93,7 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   if (e != hipSuccess) return e;
   a.v.levels = true;
   // (level, index) keys live in one 64-bit array between visits; the
-  // separate level and index outputs are written alongside
+  // index output is written alongside (the levels in out are final)
   e = hipMallocAsync((void**)&a.v.keys, (size_t)nx * ny * 8, stream);
   if (e != hipSuccess) return e;
   e = dispatch_fill<kModeSpill>(a, ts, stream, stats);

@@ -449,6 +449,134 @@ struct FillLds {
   unsigned bits;
 };
 
+// Stages the keys and kinds of a spill tile + halo (kModeSpill). NaN and
+// outside cells become inert; on the first visit the start keys are made
+// from the levels in state (bc cells: their own index). Full-width tiles
+// of grids with nx % 4 == 0 load four cells per lane (a 16-byte raw load
+// and two 16-byte key loads); every load of a batch is issued before any
+// value is used.
+template <class G>
+__device__ inline void stage_spill(const FillView& v, unsigned long long* sf, unsigned char* sd,
+                                   long long x0, long long y0, bool first) {
+  using T = unsigned long long;
+  // opaque per call: keeps the compiler from computing every batch's cell
+  // addresses once, outside the persistent loop, and spilling them
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  auto store = [&](int r, int c, float raw, unsigned char b, float lev, T key) {
+    T f = KeyLevel::kInert;  // NaN and outside cells: never lowered, never lower
+    unsigned char kind = kInertCell;
+    if (raw == raw) {  // false for outside cells too
+      // every key's level half is the cell's fill level (the levels were
+      // staged as the start keys on the first visit)
+      if (first)
+        f = (b == 1) ? KeyLevel::make(raw, (unsigned)((y0 + r) * v.nx + x0 + c))
+                     : KeyLevel::make(lev, 0xffffffffu);
+      else
+        f = key;
+      kind = (unsigned)(f >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
+    }
+    sf[G::at(r, c)] = f;
+    sd[G::at(r, c)] = kind;
+  };
+  auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
+  const bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx) && al(v.src, 15) &&
+                   (first ? (al(v.bc, 3) && al(v.state, 15)) : al(v.keys, 15));
+  if (vec) {
+    // quads of the tile's columns 0..TX-1 over rows -1..TY, in two batches
+    constexpr int Q = G::TX / 4, NV = G::ROWS * Q;
+    constexpr int UV = (NV + 2 * G::THREADS - 1) / (2 * G::THREADS);
+    for (int base = 0; base < NV; base += G::THREADS * UV) {
+      float4 raw[UV], lev[UV];
+      ulonglong2 k0[UV], k1[UV];
+      uchar4 b[UV];
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const long long gy = y0 + idx / Q - 1;
+        const float nan = __builtin_nanf("");
+        raw[u] = make_float4(nan, nan, nan, nan);
+        lev[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        b[u] = make_uchar4(0, 0, 0, 0);
+        k0[u] = k1[u] = make_ulonglong2(0ull, 0ull);
+        if (idx < NV && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + x0 + 4 * (idx % Q);
+          raw[u] = *reinterpret_cast<const float4*>(v.src + g);
+          if (first) {
+            b[u] = *reinterpret_cast<const uchar4*>(v.bc + g);
+            lev[u] = *reinterpret_cast<const float4*>(v.state + g);
+          } else {
+            k0[u] = *reinterpret_cast<const ulonglong2*>(v.keys + g);
+            k1[u] = *reinterpret_cast<const ulonglong2*>(v.keys + g + 2);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < NV) {
+          const int r = idx / Q - 1, c = 4 * (idx % Q);
+          store(r, c, raw[u].x, b[u].x, lev[u].x, k0[u].x);
+          store(r, c + 1, raw[u].y, b[u].y, lev[u].y, k0[u].y);
+          store(r, c + 2, raw[u].z, b[u].z, lev[u].z, k1[u].x);
+          store(r, c + 3, raw[u].w, b[u].w, lev[u].w, k1[u].y);
+        }
+      }
+    }
+    // halo columns -1 and TX
+    for (int idx = tid; idx < 2 * G::ROWS; idx += G::THREADS) {
+      const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      float raw = __builtin_nanf(""), lev = 0.f;
+      unsigned char b = 0;
+      T key = 0ull;
+      if (gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+        const size_t g = (size_t)gy * v.nx + gx;
+        raw = v.src[g];
+        if (first) {
+          b = v.bc[g];
+          lev = v.state[g];
+        } else {
+          key = v.keys[g];
+        }
+      }
+      store(r, c, raw, b, lev, key);
+    }
+  } else {
+    constexpr int E = G::ROWS * (G::TX + 2), U = 8;
+    for (int base = 0; base < E; base += G::THREADS * U) {
+      float raw[U], lev[U];
+      T key[U];
+      unsigned char b[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const long long gx = x0 + idx % (G::TX + 2) - 1, gy = y0 + idx / (G::TX + 2) - 1;
+        raw[u] = __builtin_nanf("");
+        lev[u] = 0.f;
+        b[u] = 0;
+        key[u] = 0ull;
+        if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + gx;
+          raw[u] = v.src[g];
+          if (first) {
+            b[u] = v.bc[g];
+            lev[u] = v.state[g];
+          } else {
+            key[u] = v.keys[g];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < E)
+          store(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1, raw[u], b[u], lev[u], key[u]);
+      }
+    }
+  }
+}
+
 // Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
@@ -469,45 +597,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   const unsigned long long c0 = wall_clock64();
   // ---- stage tile + halo
   if constexpr (MODE == kModeSpill) {
-    constexpr int E = G::ROWS * (G::TX + 2);
-    constexpr int U = 8;
-    for (int base = 0; base < E; base += G::THREADS * U) {
-      T fv[U];
-      unsigned char kv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        const int r = idx / (G::TX + 2) - 1;
-        const int c = idx % (G::TX + 2) - 1;
-        const long long gx = x0 + c, gy = y0 + r;
-        T f = L::kInert;  // NaN and outside cells: never lowered, never lower
-        unsigned char kind = kInertCell;
-        if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
-          const size_t g = (size_t)gy * v.nx + gx;
-          const float raw = v.src[g];
-          if (raw == raw) {
-            const T d = KeyLevel::make(raw, (unsigned)g);
-            // every key's level half is the cell's fill level (the levels
-            // were staged as the start keys on the first visit)
-            f = first ? ((v.bc[g] == 1) ? d : KeyLevel::make(v.state[g], 0xffffffffu))
-                      : v.keys[g];
-            kind = (unsigned)(f >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
-          }
-        }
-        fv[u] = f;
-        kv[u] = kind;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        if (idx < E) {
-          const int r = idx / (G::TX + 2) - 1;
-          const int c = idx % (G::TX + 2) - 1;
-          sf[G::at(r, c)] = fv[u];
-          sd[G::at(r, c)] = kv[u];
-        }
-      }
-    }
+    stage_spill<G>(v, sf, sd, x0, y0, first);
   } else {
     // raw inputs first (bound source in sd; bc flag or stored state in sf),
     // then one LDS pass turns them into (level, bound)
@@ -634,9 +724,10 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
           inert = sd[G::at(r, c)] == L::kInert;
         if constexpr (MODE == kModeSpill) {
           // one aligned 64-bit store: a neighbour reading its halo
-          // concurrently sees the old or the new key, never a mix
+          // concurrently sees the old or the new key, never a mix. The
+          // levels in state stay as they are: a key's level half starts
+          // at the cell's fill level and no relaxation step changes it.
           v.keys[g] = val;
-          v.state[g] = inert ? v.src[g] : unord_bits((unsigned)(val >> 32));
           v.spill[g] = inert ? -1 : (int)(unsigned)val;
         } else {
           float out = unord_bits(val);
