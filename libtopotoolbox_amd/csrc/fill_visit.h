@@ -577,6 +577,83 @@ __device__ inline void stage_spill(const FillView& v, unsigned long long* sf, un
   }
 }
 
+// Writes a relaxed tile back (cells inside the grid). Inert cells (NaN
+// input) take the input bits; kModeSpill writes the keys and the spill
+// indices and leaves the levels, which never change, as they are (one
+// aligned store per key: a neighbour reading its halo concurrently sees
+// the old or the new key, never a mix). Full-width tiles of grids with
+// nx % 4 == 0 store four cells per lane with 16-byte stores.
+template <class G, int MODE>
+__device__ inline void write_tile(const FillView& v, const typename FillLevel<MODE>::T* sf,
+                                  const typename FillLds<G, MODE>::B* sd, long long x0,
+                                  long long y0) {
+  using L = FillLevel<MODE>;
+  using T = typename L::T;
+  // opaque per call, as in stage_spill: no per-thread addresses hoisted
+  // out of the persistent loop
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  auto inert_at = [&](int at) {
+    if constexpr (MODE == kModeSpill)
+      return sd[at] == kInertCell;
+    else
+      return sd[at] == L::kInert;
+  };
+  // fill / recon output of one cell
+  auto level_out = [&](int at, size_t g) {
+    float out = unord_bits(sf[at]);
+    if (inert_at(at)) {
+      const float raw = v.src[g];
+      if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
+    }
+    return (MODE == kModeRecon) ? -out : out;
+  };
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx);
+  if constexpr (MODE == kModeSpill)
+    vec = vec && al(v.keys) && al(v.spill);
+  else
+    vec = vec && al(v.state);
+  if (vec) {
+    constexpr int Q = G::TX / 4, NQ = G::TY * Q;
+#pragma unroll 4
+    for (int q = tid; q < NQ; q += G::THREADS) {
+      const int r = q / Q, c = 4 * (q % Q);
+      const long long gy = y0 + r;
+      if (gy >= v.ny) continue;
+      const size_t g = (size_t)gy * v.nx + x0 + c;
+      const int at = G::at(r, c);
+      if constexpr (MODE == kModeSpill) {
+        const T k0 = sf[at], k1 = sf[at + 1], k2 = sf[at + 2], k3 = sf[at + 3];
+        *reinterpret_cast<ulonglong2*>(v.keys + g) = make_ulonglong2(k0, k1);
+        *reinterpret_cast<ulonglong2*>(v.keys + g + 2) = make_ulonglong2(k2, k3);
+        *reinterpret_cast<int4*>(v.spill + g) =
+            make_int4(inert_at(at) ? -1 : (int)(unsigned)k0, inert_at(at + 1) ? -1 : (int)(unsigned)k1,
+                      inert_at(at + 2) ? -1 : (int)(unsigned)k2, inert_at(at + 3) ? -1 : (int)(unsigned)k3);
+      } else {
+        *reinterpret_cast<float4*>(v.state + g) =
+            make_float4(level_out(at, g), level_out(at + 1, g + 1), level_out(at + 2, g + 2),
+                        level_out(at + 3, g + 3));
+      }
+    }
+  } else {
+    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
+      const int r = idx / G::TX, c = idx % G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx >= v.nx || gy >= v.ny) continue;
+      const size_t g = (size_t)gy * v.nx + gx;
+      const int at = G::at(r, c);
+      if constexpr (MODE == kModeSpill) {
+        const T val = sf[at];
+        v.keys[g] = val;
+        v.spill[g] = inert_at(at) ? -1 : (int)(unsigned)val;
+      } else {
+        v.state[g] = level_out(at, g);
+      }
+    }
+  }
+}
+
 // Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
@@ -710,36 +787,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // ---- write back: the first visit initialises the tile's state; later
   // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
   // input) take the input bits.
-  if (first || any_change) {
-    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-      const int r = idx / G::TX, c = idx % G::TX;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx < v.nx && gy < v.ny) {
-        const size_t g = (size_t)gy * v.nx + gx;
-        const T val = sf[G::at(r, c)];
-        bool inert;
-        if constexpr (MODE == kModeSpill)
-          inert = sd[G::at(r, c)] == kInertCell;
-        else
-          inert = sd[G::at(r, c)] == L::kInert;
-        if constexpr (MODE == kModeSpill) {
-          // one aligned 64-bit store: a neighbour reading its halo
-          // concurrently sees the old or the new key, never a mix. The
-          // levels in state stay as they are: a key's level half starts
-          // at the cell's fill level and no relaxation step changes it.
-          v.keys[g] = val;
-          v.spill[g] = inert ? -1 : (int)(unsigned)val;
-        } else {
-          float out = unord_bits(val);
-          if (inert) {
-            const float raw = v.src[g];
-            if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
-          }
-          v.state[g] = (MODE == kModeRecon) ? -out : out;
-        }
-      }
-    }
-  }
+  if (first || any_change) write_tile<G, MODE>(v, sf, sd, x0, y0);
   // ---- changed perimeter -> neighbour tiles whose cells it may lower
   unsigned bits = block_or(pushes_from_perimeter<G, true>(lds.sper, sf,
                                                           [&](int p, int q) {
