@@ -577,6 +577,114 @@ __device__ inline void stage_spill(const FillView& v, unsigned long long* sf, un
   }
 }
 
+// Stages the (level, bound) order words of a fill / reconstruction tile +
+// halo, converted in registers:
+//   kModeFill:  d = dem, f = (bc == 1) ? d : +INF on the first visit, else
+//               the stored state;
+//   kModeRecon: d = -mask, f = max(d, -marker) (a NaN marker acts as -INF);
+// NaN and outside cells are inert (+INF, +INF). Full-width tiles of grids
+// with nx % 4 == 0 load four cells per lane and array (16-byte loads; bc
+// 4 bytes) in two batches; every load of a batch is issued before any
+// value is used.
+template <class G, int MODE>
+__device__ inline void stage_levels(const FillView& v, unsigned* sf, unsigned* sd, long long x0,
+                                    long long y0, bool first) {
+  // opaque per call, as in stage_spill
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const bool use_bc = MODE == kModeFill && first;
+  const float nan = __builtin_nanf("");
+  // s: the bc flag (first fillsinks visit) or the stored state
+  auto put = [&](int r, int c, float d, float s) {
+    float f;
+    if (MODE == kModeFill) {
+      f = use_bc ? ((s == 1.f) ? d : kInf) : s;
+    } else {
+      d = -d;
+      f = fmaxf(d, (s == s) ? -s : kInf);
+    }
+    if (d != d) {
+      d = kInf;
+      f = kInf;
+    }
+    sd[G::at(r, c)] = ord_bits(d);
+    sf[G::at(r, c)] = ord_bits(f);
+  };
+  const float s_oob = use_bc ? 0.f : kInf;
+  auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
+  const bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx) && al(v.src, 15) &&
+                   (use_bc ? al(v.bc, 3) : al(v.state, 15));
+  if (vec) {
+    constexpr int Q = G::TX / 4, NV = G::ROWS * Q;
+    constexpr int UV = (NV + 2 * G::THREADS - 1) / (2 * G::THREADS);
+    for (int base = 0; base < NV; base += G::THREADS * UV) {
+      float4 dv[UV], sv[UV];
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const long long gy = y0 + idx / Q - 1;
+        dv[u] = make_float4(nan, nan, nan, nan);
+        sv[u] = make_float4(s_oob, s_oob, s_oob, s_oob);
+        if (idx < NV && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + x0 + 4 * (idx % Q);
+          dv[u] = *reinterpret_cast<const float4*>(v.src + g);
+          if (use_bc) {
+            const uchar4 b = *reinterpret_cast<const uchar4*>(v.bc + g);
+            sv[u] = make_float4(b.x, b.y, b.z, b.w);
+          } else {
+            sv[u] = *reinterpret_cast<const float4*>(v.state + g);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < NV) {
+          const int r = idx / Q - 1, c = 4 * (idx % Q);
+          put(r, c, dv[u].x, sv[u].x);
+          put(r, c + 1, dv[u].y, sv[u].y);
+          put(r, c + 2, dv[u].z, sv[u].z);
+          put(r, c + 3, dv[u].w, sv[u].w);
+        }
+      }
+    }
+    // halo columns -1 and TX
+    for (int idx = tid; idx < 2 * G::ROWS; idx += G::THREADS) {
+      const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      float d = nan, sv = s_oob;
+      if (gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+        const size_t g = (size_t)gy * v.nx + gx;
+        d = v.src[g];
+        sv = use_bc ? (float)v.bc[g] : v.state[g];
+      }
+      put(r, c, d, sv);
+    }
+  } else {
+    constexpr int E = G::ROWS * (G::TX + 2), U = 8;
+    for (int base = 0; base < E; base += G::THREADS * U) {
+      float dv[U], sv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        const long long gx = x0 + idx % (G::TX + 2) - 1, gy = y0 + idx / (G::TX + 2) - 1;
+        dv[u] = nan;
+        sv[u] = s_oob;
+        if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + gx;
+          dv[u] = v.src[g];
+          sv[u] = use_bc ? (float)v.bc[g] : v.state[g];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int idx = base + u * G::THREADS + tid;
+        if (idx < E) put(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1, dv[u], sv[u]);
+      }
+    }
+  }
+}
+
 // Writes a relaxed tile back (cells inside the grid). Inert cells (NaN
 // input) take the input bits; kModeSpill writes the keys and the spill
 // indices and leaves the levels, which never change, as they are (one
@@ -676,32 +784,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   if constexpr (MODE == kModeSpill) {
     stage_spill<G>(v, sf, sd, x0, y0, first);
   } else {
-    // raw inputs first (bound source in sd; bc flag or stored state in sf),
-    // then one LDS pass turns them into (level, bound)
-    // floats staged in place, then turned into ord_bits words
-    float* sdf = reinterpret_cast<float*>(sd);
-    float* sff = reinterpret_cast<float*>(sf);
-    const StageF32 src{v.src, sdf, __builtin_nanf("")};
-    if (MODE == kModeFill && first)
-      stage_tile<G>(src, StageU8{v.bc, sff, 0.f}, true, x0, y0, v.nx, v.ny);
-    else
-      stage_tile<G>(src, StageF32{v.state, sff, kInf}, true, x0, y0, v.nx, v.ny);
-    __syncthreads();
-    for (int k = tid; k < G::CELLS; k += G::THREADS) {
-      float d = sdf[k], f = sff[k];
-      if (MODE == kModeFill) {
-        if (first) f = (f == 1.f) ? d : kInf;
-      } else {
-        d = -d;
-        f = fmaxf(d, (f == f) ? -f : kInf);
-      }
-      if (d != d) {  // NaN input: inert cell
-        d = kInf;
-        f = kInf;
-      }
-      sd[k] = ord_bits(d);
-      sf[k] = ord_bits(f);
-    }
+    stage_levels<G, MODE>(v, sf, sd, x0, y0, first);
   }
   __syncthreads();
   // halo cells whose owner tile's stored state availability differs from
