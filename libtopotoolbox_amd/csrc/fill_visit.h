@@ -368,7 +368,8 @@ constexpr unsigned kNoNbr = 0xffffffffu;  // above every level: min() ignores it
 
 template <class G, bool VERT, bool FWD>
 __device__ __forceinline__ bool ord_sweep_regs(unsigned (&f)[64], const unsigned* sf,
-                                               const unsigned* sd, int R0, int C0, int lane) {
+                                               const unsigned (&dl)[64], int R0, int C0,
+                                               int lane) {
   // start value: the outside row (vertical) or column (horizontal)
   const int out_k = FWD ? -1 : 64;
   unsigned pc0 = VERT ? sf[G::at(R0 + out_k, C0 + lane)] : sf[G::at(R0 + lane, C0 + out_k)];
@@ -378,16 +379,13 @@ __device__ __forceinline__ bool ord_sweep_regs(unsigned (&f)[64], const unsigned
     const unsigned lo = sf[G::at(R0 + out_k, C0 - 1)], hi = sf[G::at(R0 + out_k, C0 + 64)];
     corner = (lane == 0) ? lo : hi;
   }
-  auto dcell = [&](int i) {
-    return VERT ? sd[G::at(R0 + i, C0 + lane)] : sd[G::at(R0 + lane, C0 + i)];
-  };
   unsigned changed = 0u;
   unsigned pc1 = f[FWD ? 31 : 32];  // second chain starts from the old split value
 #pragma unroll
   for (int s = 0; s < 32; s++) {
     const int i0 = FWD ? s : 63 - s;         // chain 0
     const int i1 = FWD ? 32 + s : 31 - s;    // chain 1
-    const unsigned d0 = dcell(i0), d1 = dcell(i1);
+    const unsigned d0 = dl[i0], d1 = dl[i1];
     const unsigned old0 = (s == 0) ? corner : kNoNbr;
     const unsigned a0 = dpp_u32(old0, pc0, 0x138), b0 = dpp_u32(old0, pc0, 0x130);
     const unsigned a1 = dpp_u32(kNoNbr, pc1, 0x138), b1 = dpp_u32(kNoNbr, pc1, 0x130);
@@ -413,17 +411,26 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
   unsigned it = 0;
   int any = __syncthreads_or(fill_violation<G, OrdLevel>(sf, sd, R0, C0, lane) ? 1 : 0);
   int any_change = 0;
+  // the bounds of the lane's cells in both layouts, fixed during the visit
+  unsigned dcol[64], drow[64];
+  if (any) {
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+      dcol[i] = sd[G::at(R0 + i, C0 + lane)];
+      drow[i] = sd[G::at(R0 + lane, C0 + i)];
+    }
+  }
   while (any && it < max_local) {
 #pragma unroll
     for (int r = 0; r < 64; r++) f[r] = sf[G::at(R0 + r, C0 + lane)];
-    bool ch = ord_sweep_regs<G, true, true>(f, sf, sd, R0, C0, lane);
-    ch |= ord_sweep_regs<G, true, false>(f, sf, sd, R0, C0, lane);
+    bool ch = ord_sweep_regs<G, true, true>(f, sf, dcol, R0, C0, lane);
+    ch |= ord_sweep_regs<G, true, false>(f, sf, dcol, R0, C0, lane);
 #pragma unroll
     for (int r = 0; r < 64; r++) sf[G::at(R0 + r, C0 + lane)] = f[r];
 #pragma unroll
     for (int c = 0; c < 64; c++) f[c] = sf[G::at(R0 + lane, C0 + c)];
-    ch |= ord_sweep_regs<G, false, true>(f, sf, sd, R0, C0, lane);
-    ch |= ord_sweep_regs<G, false, false>(f, sf, sd, R0, C0, lane);
+    ch |= ord_sweep_regs<G, false, true>(f, sf, drow, R0, C0, lane);
+    ch |= ord_sweep_regs<G, false, false>(f, sf, drow, R0, C0, lane);
 #pragma unroll
     for (int c = 0; c < 64; c++) sf[G::at(R0 + lane, C0 + c)] = f[c];
     any = __syncthreads_or(ch ? 1 : 0);
