@@ -240,13 +240,7 @@ __device__ inline unsigned adj_visit(const AdjView& v, long long x0, long long y
     if (any) any = __syncthreads_or(adj_violation<G>(sl, sg, ssx, ssy, R0, C0, lane) ? 1 : 0);
   }
   const unsigned long long c2 = wall_clock64();
-  if (first || any_change) {
-    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-      const int r = idx / G::TX, c = idx % G::TX;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx < v.nx && gy < v.ny) v.lam[(size_t)gy * v.nx + gx] = sl[G::at(r, c)];
-    }
-  }
+  if (first || any_change) write_tile_f32<G>(v.lam, sl, x0, y0, v.nx, v.ny);
   unsigned bits = block_or(changed_perimeter<G, false>(lds.sper, sl_bits), &lds.bits);
   if (any) bits |= kSelf;
   if (tid == 0) visit_stats(sync, it, c0, c1, c2);

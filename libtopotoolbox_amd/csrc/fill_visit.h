@@ -625,22 +625,26 @@ __device__ inline void stage_levels(const FillView& v, unsigned* sf, unsigned* s
     constexpr int Q = G::TX / 4, NV = G::ROWS * Q;
     constexpr int UV = (NV + 2 * G::THREADS - 1) / (2 * G::THREADS);
     for (int base = 0; base < NV; base += G::THREADS * UV) {
+      // raw loads only here; conversions at the LDS stores (a conversion
+      // inside the bounds branch would make each load wait there)
+      // defaults (outside cells) are the values to stage; a bc flag of 0
+      // stages as s_oob does on first visits
       float4 dv[UV], sv[UV];
+      uchar4 bv[UV];
 #pragma unroll
       for (int u = 0; u < UV; u++) {
         const int idx = base + u * G::THREADS + tid;
         const long long gy = y0 + idx / Q - 1;
         dv[u] = make_float4(nan, nan, nan, nan);
         sv[u] = make_float4(s_oob, s_oob, s_oob, s_oob);
+        bv[u] = make_uchar4(0, 0, 0, 0);
         if (idx < NV && gy >= 0 && gy < v.ny) {
           const size_t g = (size_t)gy * v.nx + x0 + 4 * (idx % Q);
           dv[u] = *reinterpret_cast<const float4*>(v.src + g);
-          if (use_bc) {
-            const uchar4 b = *reinterpret_cast<const uchar4*>(v.bc + g);
-            sv[u] = make_float4(b.x, b.y, b.z, b.w);
-          } else {
+          if (use_bc)
+            bv[u] = *reinterpret_cast<const uchar4*>(v.bc + g);
+          else
             sv[u] = *reinterpret_cast<const float4*>(v.state + g);
-          }
         }
       }
 #pragma unroll
@@ -648,10 +652,12 @@ __device__ inline void stage_levels(const FillView& v, unsigned* sf, unsigned* s
         const int idx = base + u * G::THREADS + tid;
         if (idx < NV) {
           const int r = idx / Q - 1, c = 4 * (idx % Q);
-          put(r, c, dv[u].x, sv[u].x);
-          put(r, c + 1, dv[u].y, sv[u].y);
-          put(r, c + 2, dv[u].z, sv[u].z);
-          put(r, c + 3, dv[u].w, sv[u].w);
+          const float4 d4 = dv[u];
+          const float4 s4 = use_bc ? make_float4(bv[u].x, bv[u].y, bv[u].z, bv[u].w) : sv[u];
+          put(r, c, d4.x, s4.x);
+          put(r, c + 1, d4.y, s4.y);
+          put(r, c + 2, d4.z, s4.z);
+          put(r, c + 3, d4.w, s4.w);
         }
       }
     }
@@ -660,33 +666,43 @@ __device__ inline void stage_levels(const FillView& v, unsigned* sf, unsigned* s
       const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
       const long long gx = x0 + c, gy = y0 + r;
       float d = nan, sv = s_oob;
+      uint8_t bv = 0;
       if (gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
         const size_t g = (size_t)gy * v.nx + gx;
         d = v.src[g];
-        sv = use_bc ? (float)v.bc[g] : v.state[g];
+        if (use_bc)
+          bv = v.bc[g];
+        else
+          sv = v.state[g];
       }
-      put(r, c, d, sv);
+      put(r, c, d, use_bc ? (float)bv : sv);
     }
   } else {
     constexpr int E = G::ROWS * (G::TX + 2), U = 8;
     for (int base = 0; base < E; base += G::THREADS * U) {
       float dv[U], sv[U];
+      uint8_t bv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int idx = base + u * G::THREADS + tid;
         const long long gx = x0 + idx % (G::TX + 2) - 1, gy = y0 + idx / (G::TX + 2) - 1;
         dv[u] = nan;
         sv[u] = s_oob;
+        bv[u] = 0;
         if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
           const size_t g = (size_t)gy * v.nx + gx;
           dv[u] = v.src[g];
-          sv[u] = use_bc ? (float)v.bc[g] : v.state[g];
+          if (use_bc)
+            bv[u] = v.bc[g];
+          else
+            sv[u] = v.state[g];
         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int idx = base + u * G::THREADS + tid;
-        if (idx < E) put(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1, dv[u], sv[u]);
+        if (idx < E)
+          put(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1, dv[u], use_bc ? (float)bv[u] : sv[u]);
       }
     }
   }

@@ -182,9 +182,9 @@ __device__ inline unsigned fsm_visit(const FsmView& v, long long x0, long long y
   // u from the DEM on the first visit, from the stored state afterwards;
   // fi = cellsize * slope (NaN outside the grid: never accepts a proposal)
   stage_tile<G>(StageF32{first ? v.dem : v.u, su, kInf},
-                StageF32{v.slopes, sfi, __builtin_nanf("")}, true, x0, y0, v.nx, v.ny);
+                StageF32Scaled{v.slopes, sfi, __builtin_nanf(""), v.cellsize}, true, x0, y0, v.nx,
+                v.ny);
   __syncthreads();
-  for (int k = tid; k < G::CELLS; k += G::THREADS) sfi[k] = v.cellsize * sfi[k];
   for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
     const long long gx = x0 + c, gy = y0 + r;
     if (gx < 0 || gx >= v.nx || gy < 0 || gy >= v.ny) return;
@@ -199,13 +199,7 @@ __device__ inline unsigned fsm_visit(const FsmView& v, long long x0, long long y
   const unsigned long long c2 = wall_clock64();
   // the first visit initialises the output; later visits write changed
   // tiles only
-  if (first || any_change) {
-    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-      const int r = idx / G::TX, c = idx % G::TX;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx < v.nx && gy < v.ny) v.u[(size_t)gy * v.nx + gx] = su[G::at(r, c)];
-    }
-  }
+  if (first || any_change) write_tile_f32<G>(v.u, su, x0, y0, v.nx, v.ny);
   // NaN cells compare as words, so they never count as changed. Every
   // Godunov proposal through p is at least u_p, so a neighbour cell q can
   // only be lowered by p if u_p < u_q.

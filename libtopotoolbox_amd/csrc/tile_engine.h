@@ -319,47 +319,82 @@ struct TileGeom {
 // Copies the tile + halo of one or two global arrays into LDS (float
 // layout of TileGeom). Cells outside the grid get `oob`. Full-width tiles
 // of grids with nx % 4 == 0 and aligned base pointers use 128-bit loads
-// (uint8: 32-bit); 8 loads per thread and
-// array are in flight before the LDS stores.
+// (uint8: 32-bit), in two batches per array. Values are loaded raw (V, S)
+// and converted (to4, cv) only at the LDS stores: a conversion next to its
+// load, inside the load's bounds branch, would make every load wait there
+// and serialise the batch.
 struct StageF32 {
+  using V = float4;
+  using S = float;
   const float* p;
   float* lds;
   float oob;
-  __device__ float4 load4(size_t g) const { return *reinterpret_cast<const float4*>(p + g); }
-  __device__ float load1(size_t g) const { return p[g]; }
+  __device__ V load4(size_t g) const { return *reinterpret_cast<const float4*>(p + g); }
+  __device__ S load1(size_t g) const { return p[g]; }
+  __device__ float4 to4(V v) const { return v; }
+  __device__ float cv(S v) const { return v; }
+  __device__ bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+};
+// float cells staged multiplied by `scale` (one rounding, as a separate
+// multiply in LDS would give)
+struct StageF32Scaled {
+  using V = float4;
+  using S = float;
+  const float* p;
+  float* lds;
+  float oob;
+  float scale;
+  __device__ V load4(size_t g) const { return *reinterpret_cast<const float4*>(p + g); }
+  __device__ S load1(size_t g) const { return p[g]; }
+  __device__ float4 to4(V v) const {
+    return make_float4(scale * v.x, scale * v.y, scale * v.z, scale * v.w);
+  }
+  __device__ float cv(S v) const { return scale * v; }
   __device__ bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 };
 // uint8 cells staged as floats (bc flags)
 struct StageU8 {
+  using V = uchar4;
+  using S = uint8_t;
   const uint8_t* p;
   float* lds;
   float oob;
-  __device__ float4 load4(size_t g) const {
-    const uchar4 v = *reinterpret_cast<const uchar4*>(p + g);
-    return make_float4(v.x, v.y, v.z, v.w);
-  }
-  __device__ float load1(size_t g) const { return p[g]; }
+  __device__ V load4(size_t g) const { return *reinterpret_cast<const uchar4*>(p + g); }
+  __device__ S load1(size_t g) const { return p[g]; }
+  __device__ float4 to4(V v) const { return make_float4(v.x, v.y, v.z, v.w); }
+  __device__ float cv(S v) const { return v; }
   __device__ bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 3) == 0; }
 };
 
 template <class G, class A, class B>
 __device__ inline void stage_tile(const A& a, const B& b, bool two, long long x0, long long y0,
                                   long long nx, long long ny) {
-  const int tid = threadIdx.x;
+  // opaque per call: keeps the compiler from computing every batch's
+  // addresses once, outside a persistent loop, and holding them
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   const bool vec = ((nx & 3) == 0) && (x0 + G::TX <= nx) && a.aligned() && (!two || b.aligned());
+  auto put = [&](int at, float fa, float fb) {
+    a.lds[at] = fa;
+    if (two) b.lds[at] = fb;
+  };
   if (vec) {
-    constexpr int Q = G::TX / 4, NV = G::ROWS * Q, UV = 8;
+    // two batches of loads
+    constexpr int Q = G::TX / 4, NV = G::ROWS * Q;
+    constexpr int UV = (NV + 2 * G::THREADS - 1) / (2 * G::THREADS);
     for (int base = 0; base < NV; base += G::THREADS * UV) {
-      float4 va[UV], vb[UV];
+      typename A::V va[UV];
+      typename B::V vb[UV];
+      bool ok[UV];
 #pragma unroll
       for (int u = 0; u < UV; u++) {
         const int idx = base + u * G::THREADS + tid;
-        const int r = idx / Q - 1, c4 = idx % Q;
-        const long long gy = y0 + r;
-        va[u] = make_float4(a.oob, a.oob, a.oob, a.oob);
-        vb[u] = make_float4(b.oob, b.oob, b.oob, b.oob);
-        if (idx < NV && gy >= 0 && gy < ny) {
-          const size_t g = (size_t)gy * nx + x0 + 4 * c4;
+        const long long gy = y0 + idx / Q - 1;
+        ok[u] = idx < NV && gy >= 0 && gy < ny;
+        va[u] = typename A::V{};
+        vb[u] = typename B::V{};
+        if (ok[u]) {
+          const size_t g = (size_t)gy * nx + x0 + 4 * (idx % Q);
           va[u] = a.load4(g);
           if (two) vb[u] = b.load4(g);
         }
@@ -368,18 +403,13 @@ __device__ inline void stage_tile(const A& a, const B& b, bool two, long long x0
       for (int u = 0; u < UV; u++) {
         const int idx = base + u * G::THREADS + tid;
         if (idx < NV) {
-          const int r = idx / Q - 1, c = 4 * (idx % Q);
-          const int at = G::at(r, c);
-          a.lds[at] = va[u].x;
-          a.lds[at + 1] = va[u].y;
-          a.lds[at + 2] = va[u].z;
-          a.lds[at + 3] = va[u].w;
-          if (two) {
-            b.lds[at] = vb[u].x;
-            b.lds[at + 1] = vb[u].y;
-            b.lds[at + 2] = vb[u].z;
-            b.lds[at + 3] = vb[u].w;
-          }
+          const int at = G::at(idx / Q - 1, 4 * (idx % Q));
+          const float4 fa = ok[u] ? a.to4(va[u]) : make_float4(a.oob, a.oob, a.oob, a.oob);
+          const float4 fb = ok[u] ? b.to4(vb[u]) : make_float4(b.oob, b.oob, b.oob, b.oob);
+          put(at, fa.x, fb.x);
+          put(at + 1, fa.y, fb.y);
+          put(at + 2, fa.z, fb.z);
+          put(at + 3, fa.w, fb.w);
         }
       }
     }
@@ -387,41 +417,69 @@ __device__ inline void stage_tile(const A& a, const B& b, bool two, long long x0
     for (int idx = tid; idx < 2 * G::ROWS; idx += G::THREADS) {
       const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
       const long long gx = x0 + c, gy = y0 + r;
-      float fa = a.oob, fb = b.oob;
-      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+      const bool ok = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+      typename A::S sa{};
+      typename B::S sb{};
+      if (ok) {
         const size_t g = (size_t)gy * nx + gx;
-        fa = a.load1(g);
-        if (two) fb = b.load1(g);
+        sa = a.load1(g);
+        if (two) sb = b.load1(g);
       }
-      a.lds[G::at(r, c)] = fa;
-      if (two) b.lds[G::at(r, c)] = fb;
+      put(G::at(r, c), ok ? a.cv(sa) : a.oob, ok ? b.cv(sb) : b.oob);
     }
   } else {
     constexpr int E = G::ROWS * (G::TX + 2), U = 8;
     for (int base = 0; base < E; base += G::THREADS * U) {
-      float fa[U], fb[U];
+      typename A::S sa[U];
+      typename B::S sb[U];
+      bool ok[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int idx = base + u * G::THREADS + tid;
-        const int r = idx / (G::TX + 2) - 1, c = idx % (G::TX + 2) - 1;
-        const long long gx = x0 + c, gy = y0 + r;
-        fa[u] = a.oob;
-        fb[u] = b.oob;
-        if (idx < E && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+        const long long gx = x0 + idx % (G::TX + 2) - 1, gy = y0 + idx / (G::TX + 2) - 1;
+        ok[u] = idx < E && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        sa[u] = typename A::S{};
+        sb[u] = typename B::S{};
+        if (ok[u]) {
           const size_t g = (size_t)gy * nx + gx;
-          fa[u] = a.load1(g);
-          if (two) fb[u] = b.load1(g);
+          sa[u] = a.load1(g);
+          if (two) sb[u] = b.load1(g);
         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int idx = base + u * G::THREADS + tid;
-        if (idx < E) {
-          const int r = idx / (G::TX + 2) - 1, c = idx % (G::TX + 2) - 1;
-          a.lds[G::at(r, c)] = fa[u];
-          if (two) b.lds[G::at(r, c)] = fb[u];
-        }
+        if (idx < E)
+          put(G::at(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1), ok[u] ? a.cv(sa[u]) : a.oob,
+              ok[u] ? b.cv(sb[u]) : b.oob);
       }
+    }
+  }
+}
+
+// Writes the tile's cells (float layout of TileGeom, no halo) that lie
+// inside the grid to p. Full-width tiles of grids with nx % 4 == 0 and an
+// aligned p store four cells per lane with 16-byte stores.
+template <class G>
+__device__ inline void write_tile_f32(float* p, const float* lds, long long x0, long long y0,
+                                      long long nx, long long ny) {
+  int tid = threadIdx.x;  // opaque per call, as in stage_tile
+  asm volatile("" : "+v"(tid));
+  if (((nx & 3) == 0) && (x0 + G::TX <= nx) && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    constexpr int Q = G::TX / 4, NQ = G::TY * Q;
+#pragma unroll 4
+    for (int q = tid; q < NQ; q += G::THREADS) {
+      const int r = q / Q, c = 4 * (q % Q);
+      if (y0 + r >= ny) continue;
+      const int at = G::at(r, c);
+      *reinterpret_cast<float4*>(p + (size_t)(y0 + r) * nx + x0 + c) =
+          make_float4(lds[at], lds[at + 1], lds[at + 2], lds[at + 3]);
+    }
+  } else {
+    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
+      const int r = idx / G::TX, c = idx % G::TX;
+      const long long gx = x0 + c, gy = y0 + r;
+      if (gx < nx && gy < ny) p[(size_t)gy * nx + gx] = lds[G::at(r, c)];
     }
   }
 }
