@@ -221,67 +221,6 @@ __device__ __forceinline__ unsigned long long surface_bits(const unsigned char* 
   return m;
 }
 
-// One directional spill sweep (as the order-domain sweeps, over 64-bit
-// keys in LDS). The bound of the lane's cell i of its line: inert cells
-// hold the inert key (all ones) themselves; otherwise (own level, own
-// index) on surface cells, nothing (0) on submerged ones.
-template <class G, bool VERT, bool FWD>
-__device__ __forceinline__ bool spill_sweep(unsigned long long* sf, unsigned long long surf,
-                                            const SpillBound& bound, int R0, int C0, int lane) {
-  using L = KeyLevel;
-  using T = typename L::T;
-  constexpr int NCH = kChains, LEN = 64 / kChains;
-  constexpr int step = FWD ? 1 : -1;
-  bool changed = false;
-  const int own = VERT ? C0 + lane : R0 + lane;      // lane's column / row
-  const int side = (lane == 0) ? own - 1 : own + 1;  // outside neighbour line
-  const int base = VERT ? R0 : C0;                   // line position 0
-  const int k0 = FWD ? base : base + 63;
-  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
-  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
-  // grid index of line position i: line0 + i * stride
-  const unsigned stride = VERT ? bound.nx : 1u;
-  const unsigned line0 = VERT ? bound.base + (unsigned)R0 * bound.nx + (unsigned)own
-                              : bound.base + (unsigned)own * bound.nx + (unsigned)C0;
-  T pc[NCH];
-  T fw[NCH][kPrefetch], ew[NCH][kPrefetch];
-#pragma unroll
-  for (int c = 0; c < NCH; c++) {
-    const int kc = k0 + c * LEN * step;
-    pc[c] = sf[cell(kc - step)];
-#pragma unroll
-    for (int i = 0; i < kPrefetch; i++) {
-      fw[c][i] = sf[cell(kc + i * step)];
-      ew[c][i] = sf[edge_cell(kc + (i - 1) * step)];
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < LEN; s++) {
-    const int slot = s % kPrefetch;
-#pragma unroll
-    for (int c = 0; c < NCH; c++) {
-      const int li = FWD ? c * LEN + s : 63 - (c * LEN + s);  // line position (static)
-      const int k = base + li;
-      const T fc = fw[c][slot], edge = ew[c][slot];
-      if (s + kPrefetch < LEN) {
-        fw[c][slot] = sf[cell(k + kPrefetch * step)];
-        ew[c][slot] = sf[edge_cell(k + (kPrefetch - 1) * step)];
-      }
-      const T dv = fc == L::kInert             ? L::kInert
-                   : ((surf >> li) & 1ull) != 0 ? ((fc & 0xffffffff00000000ull) |
-                                                  (line0 + (unsigned)li * stride))
-                                                : 0ull;
-      const T a = L::from_below(pc[c], edge);
-      const T b = L::from_above(pc[c], edge);
-      const T nv = L::hi(dv, L::lo(L::lo(a, b), L::lo(pc[c], fc)));
-      changed |= L::less(nv, fc);
-      pc[c] = L::lo(nv, fc);
-      sf[cell(k)] = pc[c];
-    }
-  }
-  return changed;
-}
-
 // Fixed-point test of a wavefront's quadrant: true (in some lane) if a
 // cell would still decrease. Lanes are columns walking the rows with a
 // three-row window in registers; the side columns come by DPP (the
@@ -317,6 +256,114 @@ __device__ __forceinline__ bool spill_violation(const unsigned long long* sf,
     emid = edn;
   }
   return v;
+}
+
+constexpr unsigned kNoNbr = 0xffffffffu;  // above every level / index: min() ignores it
+
+// ---------------------------------------------------------------- spill index sweeps
+// With the levels fixed, a key's relaxation only moves its index half, and
+// only between neighbours of equal level: a lower neighbour gives a key
+// below every bound (such surface cells sit at their own index from their
+// first visit on), a higher one a key above the cell's own. The register
+// sweeps below therefore relax 32-bit indices as the order-domain sweeps
+// relax levels, a neighbour counting only where a per-visit mask says its
+// level equals the cell's. Inert cells hold all ones and differ in level
+// (all ones) from every real cell, so they neither move nor spread.
+//
+// Masks of a lane's 64 cells in the column (VERT: cell i = (R0 + i,
+// C0 + lane)) or row layout (cell i = (R0 + lane, C0 + i)): byte i of
+// m[i / 4] (from bit 8 * (i % 4)) has a bit SET where the neighbour does
+// NOT count: bit 0 (i - 1, lane), 1 (i - 1, lane - 1), 2 (i - 1, lane + 1),
+// 3 (i + 1, lane), 4 (i + 1, lane - 1), 5 (i + 1, lane + 1), lane +-1
+// meaning the neighbouring line (the edge lanes' outside line from LDS).
+// Levels are the high words of the LDS keys, which a visit never changes.
+template <class G, bool VERT>
+__device__ __forceinline__ void level_masks(unsigned (&m)[16], const unsigned long long* sf,
+                                            int R0, int C0, int lane) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(sf);  // w[2 at + 1]: level
+  const int own = VERT ? C0 + lane : R0 + lane;
+  const int side = (lane == 0) ? own - 1 : own + 1;
+  auto lvl = [&](int i, int line) {
+    return VERT ? w[2 * G::at(R0 + i, line) + 1] : w[2 * G::at(line, C0 + i) + 1];
+  };
+  // line position k: own level v, lane - 1's lo, lane + 1's hi
+  unsigned pv = lvl(-1, own);
+  unsigned pe = lvl(-1, side);
+  unsigned plo = dpp_u32(pe, pv, 0x138), phi = dpp_u32(pe, pv, 0x130);
+  unsigned cv = lvl(0, own);
+  unsigned ce = lvl(0, side);
+  unsigned clo = dpp_u32(ce, cv, 0x138), chi = dpp_u32(ce, cv, 0x130);
+#pragma unroll
+  for (int i = 0; i < 16; i++) m[i] = 0u;
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    const unsigned nv = lvl(i + 1, own), ne = lvl(i + 1, side);
+    const unsigned nlo = dpp_u32(ne, nv, 0x138), nhi = dpp_u32(ne, nv, 0x130);
+    const unsigned bits = (unsigned)(pv != cv) | ((unsigned)(plo != cv) << 1) |
+                          ((unsigned)(phi != cv) << 2) | ((unsigned)(nv != cv) << 3) |
+                          ((unsigned)(nlo != cv) << 4) | ((unsigned)(nhi != cv) << 5);
+    m[i >> 2] |= bits << (8 * (i & 3));
+    pv = cv;
+    plo = clo;
+    phi = chi;
+    cv = nv;
+    clo = nlo;
+    chi = nhi;
+  }
+}
+
+// One directional index sweep over the lane's 64 cells x (as
+// ord_sweep_regs: two interleaved chains, the carried value the updated
+// cell behind, its side neighbours by DPP, the outside line and corner
+// cells from LDS). Cell i's bound is its own grid index line0 + i * stride
+// on surface cells (bit i of surf), 0 elsewhere. Returns whether a cell
+// decreased.
+template <class G, bool VERT, bool FWD>
+__device__ __forceinline__ bool spill_sweep_regs(unsigned (&x)[64], const unsigned long long* sf,
+                                                 const unsigned (&m)[16], unsigned long long surf,
+                                                 unsigned line0, unsigned stride, int R0, int C0,
+                                                 int lane) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(sf);  // w[2 at]: index
+  const int out_k = FWD ? -1 : 64;
+  unsigned pc0 = VERT ? w[2 * G::at(R0 + out_k, C0 + lane)] : w[2 * G::at(R0 + lane, C0 + out_k)];
+  unsigned corner = kNoNbr;
+  if (VERT) {
+    const unsigned lo = w[2 * G::at(R0 + out_k, C0 - 1)], hi = w[2 * G::at(R0 + out_k, C0 + 64)];
+    corner = (lane == 0) ? lo : hi;
+  }
+  constexpr int sh = FWD ? 0 : 3;  // mask bits of the cells behind
+  unsigned pc1 = x[FWD ? 31 : 32];
+  // grid indices of the two chains' current cells
+  unsigned g0 = line0 + (unsigned)(FWD ? 0 : 63) * stride;
+  unsigned g1 = line0 + (unsigned)(FWD ? 32 : 31) * stride;
+  const unsigned dg = FWD ? stride : 0u - stride;
+  auto relax = [&](int i, unsigned g, unsigned pc, unsigned a, unsigned b) {
+    const unsigned mb = m[i >> 2] >> (8 * (i & 3) + sh);
+    const unsigned cn = pc | (0u - (mb & 1u));
+    const unsigned ca = a | (0u - ((mb >> 1) & 1u));
+    const unsigned cb = b | (0u - ((mb >> 2) & 1u));
+    const unsigned mm = min(min(ca, cb), min(cn, x[i]));
+    const unsigned bnd = g & (0u - (unsigned)((surf >> i) & 1ull));
+    return max(bnd, mm);
+  };
+  unsigned changed = 0u;
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+    const int i0 = FWD ? s : 63 - s;
+    const int i1 = FWD ? 32 + s : 31 - s;
+    const unsigned old0 = (s == 0) ? corner : kNoNbr;
+    const unsigned a0 = dpp_u32(old0, pc0, 0x138), b0 = dpp_u32(old0, pc0, 0x130);
+    const unsigned a1 = dpp_u32(kNoNbr, pc1, 0x138), b1 = dpp_u32(kNoNbr, pc1, 0x130);
+    const unsigned n0 = relax(i0, g0, pc0, a0, b0), n1 = relax(i1, g1, pc1, a1, b1);
+    changed |= (unsigned)(n0 < x[i0]) | (unsigned)(n1 < x[i1]);
+    x[i0] = n0;
+    x[i1] = n1;
+    pc0 = n0;
+    pc1 = n1;
+    g0 += dg;
+    g1 += dg;
+  }
+  return changed != 0u;
 }
 
 // The same test for the fill / reconstruction order domain (bounds in sd).
@@ -364,7 +411,6 @@ __device__ __forceinline__ bool fill_violation(const typename L::T* sf, const ty
 // Each sweep runs as two interleaved chains of 32 steps (the second starts
 // from the old value at the split) so one chain's dependent DPP / min /
 // med3 sequence overlaps the other's.
-constexpr unsigned kNoNbr = 0xffffffffu;  // above every level: min() ignores it
 
 template <class G, bool VERT, bool FWD>
 __device__ __forceinline__ bool ord_sweep_regs(unsigned (&f)[64], const unsigned* sf,
@@ -875,11 +921,37 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
     const unsigned long long vsurf = surface_bits<G, true>(bound.kind, R0, C0, lane);
     const unsigned long long hsurf = surface_bits<G, false>(bound.kind, R0, C0, lane);
     any = __syncthreads_or(spill_violation<G>(sf, vsurf, bound, R0, C0, lane) ? 1 : 0);
+    // indices in registers (spill_sweep_regs); the keys' index halves in
+    // LDS are updated after each layout's sweeps
+    unsigned* w = reinterpret_cast<unsigned*>(sf);
+    unsigned x[64], mcol[16], mrow[16];
+    if (any) {
+      level_masks<G, true>(mcol, sf, R0, C0, lane);
+      level_masks<G, false>(mrow, sf, R0, C0, lane);
+    }
+    const unsigned vline0 = bound.base + (unsigned)R0 * bound.nx + (unsigned)(C0 + lane);
+    const unsigned hline0 = bound.base + (unsigned)(R0 + lane) * bound.nx + (unsigned)C0;
     while (any && it < max_local) {
-      bool ch = spill_sweep<G, true, true>(sf, vsurf, bound, R0, C0, lane);
-      ch |= spill_sweep<G, true, false>(sf, vsurf, bound, R0, C0, lane);
-      ch |= spill_sweep<G, false, true>(sf, hsurf, bound, R0, C0, lane);
-      ch |= spill_sweep<G, false, false>(sf, hsurf, bound, R0, C0, lane);
+      // opaque per iteration: the per-step mask bits and bounds are
+      // recomputed in the sweeps instead of being hoisted out of this loop
+      // (about a thousand registers' worth)
+      unsigned long long vs = vsurf, hs = hsurf;
+      unsigned vl = vline0, hl = hline0;
+      asm volatile("" : "+v"(vs), "+v"(hs), "+v"(vl), "+v"(hl));
+#pragma unroll
+      for (int k = 0; k < 16; k++) asm volatile("" : "+v"(mcol[k]), "+v"(mrow[k]));
+#pragma unroll
+      for (int r = 0; r < 64; r++) x[r] = w[2 * G::at(R0 + r, C0 + lane)];
+      bool ch = spill_sweep_regs<G, true, true>(x, sf, mcol, vs, vl, bound.nx, R0, C0, lane);
+      ch |= spill_sweep_regs<G, true, false>(x, sf, mcol, vs, vl, bound.nx, R0, C0, lane);
+#pragma unroll
+      for (int r = 0; r < 64; r++) w[2 * G::at(R0 + r, C0 + lane)] = x[r];
+#pragma unroll
+      for (int c = 0; c < 64; c++) x[c] = w[2 * G::at(R0 + lane, C0 + c)];
+      ch |= spill_sweep_regs<G, false, true>(x, sf, mrow, hs, hl, 1u, R0, C0, lane);
+      ch |= spill_sweep_regs<G, false, false>(x, sf, mrow, hs, hl, 1u, R0, C0, lane);
+#pragma unroll
+      for (int c = 0; c < 64; c++) w[2 * G::at(R0 + lane, C0 + c)] = x[c];
       any = __syncthreads_or(ch ? 1 : 0);
       any_change |= any;
       it++;
