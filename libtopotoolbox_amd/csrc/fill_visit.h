@@ -447,6 +447,34 @@ __device__ __forceinline__ bool ord_sweep_regs(unsigned (&f)[64], const unsigned
   return changed != 0u;
 }
 
+// Fixed-point test of a wavefront's quadrant from the row layout held in
+// registers (f[i]: cell (R0 + lane, C0 + i), d the bounds of the same
+// cells): lanes are rows walking the columns; the rows above / below come
+// by DPP, the edge lanes' outside row and every lane's outside columns
+// from LDS. True (in some lane) if a cell would still decrease.
+template <class G>
+__device__ __forceinline__ bool fill_violation_rows(const unsigned (&f)[64], const unsigned (&d)[64],
+                                                    const unsigned* sf, int R0, int C0, int lane) {
+  const int R = R0 + lane;
+  const int er = (lane == 0) ? R0 - 1 : R0 + 64;  // outside row for the edge lanes
+  const unsigned left = sf[G::at(R, C0 - 1)], right = sf[G::at(R, C0 + 64)];
+  unsigned ep = sf[G::at(er, C0 - 1)], ec = sf[G::at(er, C0)];
+  bool v = false;
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    const unsigned op = (i == 0) ? left : f[i - 1];
+    const unsigned on = (i == 63) ? right : f[i + 1];
+    const unsigned en = sf[G::at(er, C0 + i + 1)];
+    const unsigned row = min(min(op, on), f[i]);  // own row, columns i-1..i+1
+    const unsigned erow = min(min(ep, en), ec);   // outside row (edge lanes)
+    const unsigned m = min(row, min(dpp_u32(erow, row, 0x138), dpp_u32(erow, row, 0x130)));
+    v |= max(d[i], m) < f[i];
+    ep = ec;
+    ec = en;
+  }
+  return v;
+}
+
 // Relaxes a tile to its local fixed point with the register sweeps; the
 // tile's state is in sf before and after. Returns the sweep iterations.
 template <class G>
@@ -482,7 +510,7 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
     any = __syncthreads_or(ch ? 1 : 0);
     any_change |= any;
     it++;
-    if (any) any = __syncthreads_or(fill_violation<G, OrdLevel>(sf, sd, R0, C0, lane) ? 1 : 0);
+    if (any) any = __syncthreads_or(fill_violation_rows<G>(f, drow, sf, R0, C0, lane) ? 1 : 0);
   }
   *any_out = any;
   *changed_out = any_change;
