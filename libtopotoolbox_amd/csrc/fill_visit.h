@@ -483,16 +483,19 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
                                           int* changed_out) {
   unsigned f[64];
   unsigned it = 0;
-  int any = __syncthreads_or(fill_violation<G, OrdLevel>(sf, sd, R0, C0, lane) ? 1 : 0);
-  int any_change = 0;
-  // the bounds of the lane's cells in both layouts, fixed during the visit
+  // the bounds of the lane's cells in both layouts, fixed during the visit;
+  // the first fixed-point test runs on the row layout
   unsigned dcol[64], drow[64];
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    drow[i] = sd[G::at(R0 + lane, C0 + i)];
+    f[i] = sf[G::at(R0 + lane, C0 + i)];
+  }
+  int any = __syncthreads_or(fill_violation_rows<G>(f, drow, sf, R0, C0, lane) ? 1 : 0);
+  int any_change = 0;
   if (any) {
 #pragma unroll
-    for (int i = 0; i < 64; i++) {
-      dcol[i] = sd[G::at(R0 + i, C0 + lane)];
-      drow[i] = sd[G::at(R0 + lane, C0 + i)];
-    }
+    for (int i = 0; i < 64; i++) dcol[i] = sd[G::at(R0 + i, C0 + lane)];
   }
   while (any && it < max_local) {
 #pragma unroll
