@@ -274,8 +274,9 @@ constexpr unsigned kNoNbr = 0xffffffffu;  // above every level / index: min() ig
 // C0 + lane)) or row layout (cell i = (R0 + lane, C0 + i)): byte i of
 // m[i / 4] (from bit 8 * (i % 4)) has a bit SET where the neighbour does
 // NOT count: bit 0 (i - 1, lane), 1 (i - 1, lane - 1), 2 (i - 1, lane + 1),
-// 3 (i + 1, lane), 4 (i + 1, lane - 1), 5 (i + 1, lane + 1), lane +-1
-// meaning the neighbouring line (the edge lanes' outside line from LDS).
+// 3 (i + 1, lane), 4 (i + 1, lane - 1), 5 (i + 1, lane + 1), 6 (i, lane - 1),
+// 7 (i, lane + 1), lane +-1 meaning the neighbouring line (the edge lanes'
+// outside line from LDS).
 // Levels are the high words of the LDS keys, which a visit never changes.
 template <class G, bool VERT>
 __device__ __forceinline__ void level_masks(unsigned (&m)[16], const unsigned long long* sf,
@@ -301,7 +302,8 @@ __device__ __forceinline__ void level_masks(unsigned (&m)[16], const unsigned lo
     const unsigned nlo = dpp_u32(ne, nv, 0x138), nhi = dpp_u32(ne, nv, 0x130);
     const unsigned bits = (unsigned)(pv != cv) | ((unsigned)(plo != cv) << 1) |
                           ((unsigned)(phi != cv) << 2) | ((unsigned)(nv != cv) << 3) |
-                          ((unsigned)(nlo != cv) << 4) | ((unsigned)(nhi != cv) << 5);
+                          ((unsigned)(nlo != cv) << 4) | ((unsigned)(nhi != cv) << 5) |
+                          ((unsigned)(clo != cv) << 6) | ((unsigned)(chi != cv) << 7);
     m[i >> 2] |= bits << (8 * (i & 3));
     pv = cv;
     plo = clo;
@@ -364,6 +366,50 @@ __device__ __forceinline__ bool spill_sweep_regs(unsigned (&x)[64], const unsign
     g1 += dg;
   }
   return changed != 0u;
+}
+
+// Fixed-point test of the index relaxation from the row layout in
+// registers (x[i]: cell (R0 + lane, C0 + i), m its masks, surf its surface
+// bits, line0 + i its grid index): own row by register, the rows above /
+// below by DPP (the edge lanes' outside row and every lane's outside
+// columns from LDS). True (in some lane) if a cell would still decrease.
+template <class G>
+__device__ __forceinline__ bool spill_violation_rows(const unsigned (&x)[64],
+                                                     const unsigned long long* sf,
+                                                     const unsigned (&m)[16],
+                                                     unsigned long long surf, unsigned line0,
+                                                     int R0, int C0, int lane) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(sf);  // w[2 at]: index
+  const int R = R0 + lane;
+  const int er = (lane == 0) ? R0 - 1 : R0 + 64;  // outside row for the edge lanes
+  auto lds = [&](int r, int c) { return w[2 * G::at(r, c)]; };
+  // position k: own value, lane - 1's (lo) and lane + 1's (hi)
+  unsigned pv = lds(R, C0 - 1);
+  unsigned pe = lds(er, C0 - 1);
+  unsigned plo = dpp_u32(pe, pv, 0x138), phi = dpp_u32(pe, pv, 0x130);
+  unsigned cv = x[0];
+  unsigned ce = lds(er, C0);
+  unsigned clo = dpp_u32(ce, cv, 0x138), chi = dpp_u32(ce, cv, 0x130);
+  bool v = false;
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    const unsigned nv = (i == 63) ? lds(R, C0 + 64) : x[i + 1];
+    const unsigned ne = lds(er, C0 + i + 1);
+    const unsigned nlo = dpp_u32(ne, nv, 0x138), nhi = dpp_u32(ne, nv, 0x130);
+    const unsigned mb = m[i >> 2] >> (8 * (i & 3));
+    auto c = [&](unsigned val, int bit) { return val | (0u - ((mb >> bit) & 1u)); };
+    const unsigned mm = min(min(min(c(pv, 0), c(plo, 1)), min(c(phi, 2), c(nv, 3))),
+                            min(min(c(nlo, 4), c(nhi, 5)), min(c(clo, 6), c(chi, 7))));
+    const unsigned bnd = (line0 + (unsigned)i) & (0u - (unsigned)((surf >> i) & 1ull));
+    v |= max(bnd, min(mm, cv)) < cv;
+    pv = cv;
+    plo = clo;
+    phi = chi;
+    cv = nv;
+    clo = nlo;
+    chi = nhi;
+  }
+  return v;
 }
 
 // The same test for the fill / reconstruction order domain (bounds in sd).
@@ -986,8 +1032,15 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       any = __syncthreads_or(ch ? 1 : 0);
       any_change |= any;
       it++;
-      if (any)
-        any = __syncthreads_or(spill_violation<G>(sf, vsurf, bound, R0, C0, lane) ? 1 : 0);
+      if (any) {
+        // opaque again: nothing derived from the masks stays live from the
+        // sweeps into the test
+        asm volatile("" : "+v"(hs), "+v"(hl));
+#pragma unroll
+        for (int k = 0; k < 16; k++) asm volatile("" : "+v"(mrow[k]));
+        any = __syncthreads_or(
+            spill_violation_rows<G>(x, sf, mrow, hs, hl, R0, C0, lane) ? 1 : 0);
+      }
     }
   } else {
     it = ord_relax_regs<G>(sf, sd, R0, C0, lane, max_local, &any, &any_change);
