@@ -196,9 +196,9 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
         const FillView v{a.buf.output + off, a.buf.spill + off, acc, a.buf.dem + off,
                          a.buf.bc + off, nx, ny, ph == 1};
         if (ph == 0)
-          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, a.max_local, a.q.sync,
+          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(),
                                           lds.level);
-        return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, a.max_local, a.q.sync,
+        return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(),
                                          lds.fill);
       }
       const int* spill = a.buf.spill + off;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       if (ph == 0) {
         const FsmView v{a.buf.output + off, a.buf.dem + off, a.buf.slopes + off, it.cellsize,
                         nx, ny};
-        return fsm_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.sync, lds.fsm);
+        return fsm_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds.fsm);
       }
       if (ph == 1) {
         for_cells([&](size_t p) {
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       }
       if (ph == 2) {
         const AdjView v{a.lam + off, a.buf.g_out + off, a.sx + off, a.sy + off, nx, ny};
-        return adj_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.sync, lds.adj);
+        return adj_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds.adj);
       }
       for_cells([&](size_t p) {
         fsm_adjoint_finish_cell(p, a.buf.g_dem + off, a.buf.g_slopes + off, a.lam + off,
@@ -440,6 +440,7 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
   a.q.state = reinterpret_cast<unsigned*>(a.q.ring + h.ntiles);
   a.q.visited = a.q.state + h.ntiles;
   a.q.ntiles = h.ntiles;
+  a.q.stats = stats ? 1 : 0;
   const size_t cells = (size_t)h.cells;
   if (h.op == kBatchFill) {
     a.acc = reinterpret_cast<unsigned long long*>(ws + h.ws_scratch);

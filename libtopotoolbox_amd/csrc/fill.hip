@@ -22,7 +22,7 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
     return;
   run_queue(a.q, GridTopo{a.ntx, a.nty}, [&](int t, bool first, unsigned nvis) {
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
-    return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.sync, lds);
+    return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds);
   });
 }
 

@@ -17,7 +17,7 @@ __global__ __launch_bounds__(G::THREADS) void fsm_persistent(FsmArgs a) {
   __shared__ FsmLds<G> lds;
   run_queue(a.q, GridTopo{a.ntx, a.nty}, [&](int t, bool first, unsigned nvis) {
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
-    return fsm_visit<G>(a.v, x0, y0, first, nvis, a.max_local, a.q.sync, lds);
+    return fsm_visit<G>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds);
   });
 }
 

@@ -74,6 +74,11 @@ struct TileQueue {
   unsigned* state;    // per tile: kQueued | kRunning bits
   unsigned* visited;  // per tile: 1 once its first visit has been written
   int ntiles;
+  // 1: count visits and time their phases in *sync (atomics each visit;
+  // only when the caller reads them back)
+  int stats;
+  // the Sync block visits report their statistics to, or null
+  __device__ Sync* stats_sink() const { return stats ? sync : nullptr; }
 };
 
 enum : unsigned { kQueued = 1u, kRunning = 2u };
@@ -225,7 +230,7 @@ __device__ inline void run_queue(const TileQueue& q, const Topo& topo, Visit vis
         }
         s_nvis = nvis;
       }
-      atomicAdd(&q.sync->t_wait, wall_clock64() - w0);
+      if (q.stats) atomicAdd(&q.sync->t_wait, wall_clock64() - w0);
     }
     __syncthreads();
     const int tt = s_tile;
@@ -280,6 +285,7 @@ __device__ inline void run_queue(const TileQueue& q, const Topo& topo, Visit vis
 // relaxed).
 __device__ inline void visit_stats(Sync* s, unsigned it, unsigned long long c0,
                                    unsigned long long c1, unsigned long long c2) {
+  if (!s) return;  // statistics not requested
   atomicAdd(&s->visits, 1ull);
   atomicAdd(&s->local_iters, (unsigned long long)it);
   atomicMax(&s->max_local, it);

@@ -45,6 +45,7 @@ hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a
   e = ws.alloc(ntiles, ntx, nty, stream);
   if (e != hipSuccess) return e;
   a.q = ws.q;
+  a.q.stats = stats ? 1 : 0;
   kern<<<grid, threads, 0, stream>>>(a);
   e = hipGetLastError();
   if (stats) {
