@@ -237,12 +237,18 @@ __device__ inline void run_queue(const TileQueue& q, const Topo& topo, Visit vis
     const bool first = s_first != 0;
     const unsigned nvis = s_nvis;
     if (tt < 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // thread 0's agent-scope acquire after the pop, ordered before every
+    // wave's loads by the barrier, covers the workgroup: the caches it
+    // invalidates (the CU's L1, the XCD's L2) are the ones all its waves
+    // read through
     const unsigned bits = visit(tt, first, nvis);
-    // publish the tile's writes before anyone can pop a tile we queue
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave's stores complete (in L2) at the barrier; one agent-scope
+    // release by thread 0 then writes back the L2 that holds them all,
+    // before anyone can pop a tile we queue
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       s_expand = kNoExpand;
       if (first) {
         __hip_atomic_store(&q.visited[tt], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
