@@ -96,57 +96,77 @@ struct AdjView {
 // Contribution of neighbour lambda l through weight w (0 when w <= 0).
 __device__ __forceinline__ float pull(float w, float l) { return w > 0.f ? w * l : 0.f; }
 
-// lambda = g + up + down + left + right, always summed in that order.
+// One directional lambda sweep: lambda = g + up + down + left + right,
+// always summed in that order, each term pulled from a neighbour whose
+// weight points at the cell. The wavefront's 64 lambdas of its line are in
+// registers (l[i]: cell (R0 + i, C0 + lane) in the column layout,
+// (R0 + lane, C0 + i) in the row layout); g, the weights and the
+// quadrant's outside line come from LDS (independent loads); the side
+// neighbours by DPP. Two interleaved chains of 32 steps, the second
+// starting from the old value at the split, as in the other register
+// sweeps: the triangular system's fixed point is reached from any update
+// order.
 template <class G, bool VERT, bool FWD>
-__device__ __forceinline__ bool adj_sweep(float* sl, const float* sg, const float* ssx,
-                                          const float* ssy, int R0, int C0, int lane) {
+__device__ __forceinline__ bool adj_sweep_regs(float (&l)[64], const float* sl, const float* sg,
+                                               const float* ssx, const float* ssy, int R0,
+                                               int C0, int lane) {
   constexpr int step = FWD ? 1 : -1;
-  bool changed = false;
   const int own = VERT ? C0 + lane : R0 + lane;
   const int side = (lane == 0) ? own - 1 : own + 1;
-  const int k0 = VERT ? (FWD ? R0 : R0 + 63) : (FWD ? C0 : C0 + 63);
-  auto cell = [&](int k) { return VERT ? G::at(k, own) : G::at(own, k); };
-  auto edge_cell = [&](int k) { return VERT ? G::at(k, side) : G::at(side, k); };
-  // weights along the sweep axis: s_along = sy for vertical sweeps, sx for
-  // horizontal ones; across the lanes: the other one
+  const int base = VERT ? R0 : C0;
+  // line position i in [-1, 64]
+  auto cell = [&](int i) { return VERT ? G::at(base + i, own) : G::at(own, base + i); };
+  auto edge_cell = [&](int i) { return VERT ? G::at(base + i, side) : G::at(side, base + i); };
   const float* s_al = VERT ? ssy : ssx;
   const float* s_ac = VERT ? ssx : ssy;
-  float pl = sl[cell(k0 - step)];    // lambda of the previous cell along the sweep
-  float ps = s_al[cell(k0 - step)];  // its weight along the sweep axis
-  int k = k0;
-#pragma unroll 2
-  for (int s = 0; s < 64; s++, k += step) {
-    const float cur = sl[cell(k)];
-    const float g = sg[cell(k)];
-    const float lf = sl[cell(k + step)];
-    const float sf = s_al[cell(k + step)];
-    const float sa = s_ac[cell(k)];
-    const float le = sl[edge_cell(k)];
-    const float se = s_ac[edge_cell(k)];
-    // across-lane neighbours: lower lane (column C-1 or row R-1), upper lane
-    const float l_lo = lane_from_below(cur, le), s_lo = lane_from_below(sa, se);
-    const float l_hi = lane_from_above(cur, le), s_hi = lane_from_above(sa, se);
-    // along-axis neighbours: "before" (index k-1) and "after" (index k+1)
-    const float l_bf = FWD ? pl : lf, s_bf = FWD ? ps : sf;
-    const float l_af = FWD ? lf : pl, s_af = FWD ? sf : ps;
-    float up, dn, lt, rt;
-    if (VERT) {  // along = y: before = up (R-1), after = down (R+1)
-      up = pull(s_bf, l_bf);
-      dn = pull(-s_af, l_af);
-      lt = pull(s_lo, l_lo);
-      rt = pull(-s_hi, l_hi);
-    } else {  // along = x: before = left, after = right; lanes are rows
-      lt = pull(s_bf, l_bf);
-      rt = pull(-s_af, l_af);
-      up = pull(s_lo, l_lo);
-      dn = pull(-s_hi, l_hi);
+  const int out_i = FWD ? -1 : 64, end_i = FWD ? 64 : -1;
+  const float beyond = sl[cell(end_i)];
+  // per chain: lambda and along-axis weight of the cell behind, and the
+  // weight of the current cell (the previous step's cell ahead)
+  float pl[2], ps[2], sc[2];
+  pl[0] = sl[cell(out_i)];
+  ps[0] = s_al[cell(out_i)];
+  sc[0] = s_al[cell(FWD ? 0 : 63)];
+  pl[1] = l[FWD ? 31 : 32];
+  ps[1] = s_al[cell(FWD ? 31 : 32)];
+  sc[1] = s_al[cell(FWD ? 32 : 31)];
+  bool changed = false;
+#pragma unroll
+  for (int s = 0; s < 32; s++) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int i = FWD ? c * 32 + s : 63 - (c * 32 + s);
+      const int j = i + step;
+      const float cur = l[i];
+      const float g = sg[cell(i)];
+      const float lf = (j >= 0 && j < 64) ? l[j] : beyond;
+      const float sf = s_al[cell(j)];
+      const float sa = s_ac[cell(i)];
+      const float le = sl[edge_cell(i)];
+      const float se = s_ac[edge_cell(i)];
+      const float l_lo = lane_from_below(cur, le), s_lo = lane_from_below(sa, se);
+      const float l_hi = lane_from_above(cur, le), s_hi = lane_from_above(sa, se);
+      const float l_bf = FWD ? pl[c] : lf, s_bf = FWD ? ps[c] : sf;
+      const float l_af = FWD ? lf : pl[c], s_af = FWD ? sf : ps[c];
+      float up, dn, lt, rt;
+      if (VERT) {
+        up = pull(s_bf, l_bf);
+        dn = pull(-s_af, l_af);
+        lt = pull(s_lo, l_lo);
+        rt = pull(-s_hi, l_hi);
+      } else {
+        lt = pull(s_bf, l_bf);
+        rt = pull(-s_af, l_af);
+        up = pull(s_lo, l_lo);
+        dn = pull(-s_hi, l_hi);
+      }
+      const float nv = (((g + up) + dn) + lt) + rt;
+      changed |= __float_as_uint(nv) != __float_as_uint(cur);
+      l[i] = nv;
+      pl[c] = nv;
+      ps[c] = sc[c];
+      sc[c] = sf;
     }
-    const float nv = (((g + up) + dn) + lt) + rt;
-    const bool upd = __float_as_uint(nv) != __float_as_uint(cur);
-    if (upd) sl[cell(k)] = nv;
-    changed |= upd;
-    pl = nv;
-    ps = s_al[cell(k)];
   }
   return changed;
 }
@@ -229,11 +249,25 @@ __device__ inline unsigned adj_visit(const AdjView& v, long long x0, long long y
   unsigned it = 0;
   int any = __syncthreads_or(adj_violation<G>(sl, sg, ssx, ssy, R0, C0, lane) ? 1 : 0);
   int any_change = 0;
+  float l[64];
   while (any && it < max_local) {
-    bool ch = adj_sweep<G, true, true>(sl, sg, ssx, ssy, R0, C0, lane);
-    ch |= adj_sweep<G, true, false>(sl, sg, ssx, ssy, R0, C0, lane);
-    ch |= adj_sweep<G, false, true>(sl, sg, ssx, ssy, R0, C0, lane);
-    ch |= adj_sweep<G, false, false>(sl, sg, ssx, ssy, R0, C0, lane);
+    // opaque per iteration: the LDS reads of g and the weights (which the
+    // sweeps never write) stay in the sweeps instead of being hoisted out
+    // of this loop into hundreds of registers
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+#pragma unroll
+    for (int r = 0; r < 64; r++) l[r] = sl[G::at(R0 + r, C0 + ln)];
+    bool ch = adj_sweep_regs<G, true, true>(l, sl, sg, ssx, ssy, R0, C0, ln);
+    ch |= adj_sweep_regs<G, true, false>(l, sl, sg, ssx, ssy, R0, C0, ln);
+#pragma unroll
+    for (int r = 0; r < 64; r++) sl[G::at(R0 + r, C0 + ln)] = l[r];
+#pragma unroll
+    for (int c = 0; c < 64; c++) l[c] = sl[G::at(R0 + ln, C0 + c)];
+    ch |= adj_sweep_regs<G, false, true>(l, sl, sg, ssx, ssy, R0, C0, ln);
+    ch |= adj_sweep_regs<G, false, false>(l, sl, sg, ssx, ssy, R0, C0, ln);
+#pragma unroll
+    for (int c = 0; c < 64; c++) sl[G::at(R0 + ln, C0 + c)] = l[c];
     any = __syncthreads_or(ch ? 1 : 0);
     any_change |= any;
     it++;
