@@ -526,7 +526,7 @@ __device__ __forceinline__ bool fill_violation_rows(const unsigned (&f)[64], con
 template <class G>
 __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int R0, int C0,
                                           int lane, unsigned max_local, int* any_out,
-                                          int* changed_out) {
+                                          int* changed_out, int* wave_changed_out) {
   unsigned f[64];
   unsigned it = 0;
   // the bounds of the lane's cells in both layouts, fixed during the visit;
@@ -538,7 +538,7 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
     f[i] = sf[G::at(R0 + lane, C0 + i)];
   }
   int any = __syncthreads_or(fill_violation_rows<G>(f, drow, sf, R0, C0, lane) ? 1 : 0);
-  int any_change = 0;
+  int any_change = 0, wave_change = 0;
   if (any) {
 #pragma unroll
     for (int i = 0; i < 64; i++) dcol[i] = sd[G::at(R0 + i, C0 + lane)];
@@ -556,6 +556,7 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
     ch |= ord_sweep_regs<G, false, false>(f, sf, drow, R0, C0, lane);
 #pragma unroll
     for (int c = 0; c < 64; c++) sf[G::at(R0 + lane, C0 + c)] = f[c];
+    wave_change |= __any(ch ? 1 : 0);
     any = __syncthreads_or(ch ? 1 : 0);
     any_change |= any;
     it++;
@@ -563,6 +564,7 @@ __device__ inline unsigned ord_relax_regs(unsigned* sf, const unsigned* sd, int 
   }
   *any_out = any;
   *changed_out = any_change;
+  *wave_changed_out = wave_change;
   return it;
 }
 
@@ -908,6 +910,44 @@ __device__ inline void write_tile(const FillView& v, const typename FillLevel<MO
   }
 }
 
+// Writes the wavefront's own 64x64 quadrant (tile rows R0.., columns C0..)
+// back, for fill / reconstruction: a quadrant the visit did not change is
+// left as it is in memory. Same values as write_tile.
+template <class G, int MODE>
+__device__ inline void write_quadrant(const FillView& v, const unsigned* sf, const unsigned* sd,
+                                      long long x0, long long y0, int R0, int C0, int lane) {
+  auto level_out = [&](int at, size_t g) {
+    float out = unord_bits(sf[at]);
+    if (sd[at] == OrdLevel::kInert) {
+      const float raw = v.src[g];
+      if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
+    }
+    return (MODE == kModeRecon) ? -out : out;
+  };
+  const long long qx = x0 + C0, qy = y0 + R0;
+  if (((v.nx & 3) == 0) && (qx + 64 <= v.nx) &&
+      (reinterpret_cast<uintptr_t>(v.state) & 15) == 0) {
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+      const int q = lane + 64 * k;
+      const int r = q >> 4, c = 4 * (q & 15);
+      if (qy + r >= v.ny) continue;
+      const size_t g = (size_t)(qy + r) * v.nx + qx + c;
+      const int at = G::at(R0 + r, C0 + c);
+      *reinterpret_cast<float4*>(v.state + g) =
+          make_float4(level_out(at, g), level_out(at + 1, g + 1), level_out(at + 2, g + 2),
+                      level_out(at + 3, g + 3));
+    }
+  } else {
+    for (int r = 0; r < 64; r++) {
+      const long long gx = qx + lane, gy = qy + r;
+      if (gx >= v.nx || gy >= v.ny) continue;
+      const size_t g = (size_t)gy * v.nx + gx;
+      v.state[g] = level_out(G::at(R0 + r, C0 + lane), g);
+    }
+  }
+}
+
 // Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
@@ -993,7 +1033,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // ---- relax: sweep iterations until the parallel fixed-point test
   // passes (also tried before the first one)
   unsigned it = 0;
-  int any = 0, any_change = 0;
+  int any = 0, any_change = 0, wave_change = 0;
   if constexpr (MODE == kModeSpill) {
     const unsigned long long vsurf = surface_bits<G, true>(bound.kind, R0, C0, lane);
     const unsigned long long hsurf = surface_bits<G, false>(bound.kind, R0, C0, lane);
@@ -1043,13 +1083,18 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       }
     }
   } else {
-    it = ord_relax_regs<G>(sf, sd, R0, C0, lane, max_local, &any, &any_change);
+    it = ord_relax_regs<G>(sf, sd, R0, C0, lane, max_local, &any, &any_change, &wave_change);
   }
   const unsigned long long c2 = wall_clock64();
   // ---- write back: the first visit initialises the tile's state; later
   // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
   // input) take the input bits.
-  if (first || any_change) write_tile<G, MODE>(v, sf, sd, x0, y0);
+  if constexpr (MODE == kModeSpill) {
+    if (first || any_change) write_tile<G, MODE>(v, sf, sd, x0, y0);
+  } else {
+    // each wavefront writes its own quadrant if its sweeps changed it
+    if (first || wave_change) write_quadrant<G, MODE>(v, sf, sd, x0, y0, R0, C0, lane);
+  }
   // ---- changed perimeter -> neighbour tiles whose cells it may lower
   unsigned bits = block_or(pushes_from_perimeter<G, true>(lds.sper, sf,
                                                           [&](int p, int q) {
