@@ -948,6 +948,43 @@ __device__ inline void write_quadrant(const FillView& v, const unsigned* sf, con
   }
 }
 
+// The spill pass's version: the wavefront's quadrant of keys and spill
+// indices (one aligned store per key, as in write_tile).
+template <class G>
+__device__ inline void write_spill_quadrant(const FillView& v, const unsigned long long* sf,
+                                            const unsigned char* sd, long long x0, long long y0,
+                                            int R0, int C0, int lane) {
+  using T = unsigned long long;
+  const long long qx = x0 + C0, qy = y0 + R0;
+  auto spill_of = [&](int at, T k) { return sd[at] == kInertCell ? -1 : (int)(unsigned)k; };
+  if (((v.nx & 3) == 0) && (qx + 64 <= v.nx) && (reinterpret_cast<uintptr_t>(v.keys) & 15) == 0 &&
+      (reinterpret_cast<uintptr_t>(v.spill) & 15) == 0) {
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+      const int q = lane + 64 * k;
+      const int r = q >> 4, c = 4 * (q & 15);
+      if (qy + r >= v.ny) continue;
+      const size_t g = (size_t)(qy + r) * v.nx + qx + c;
+      const int at = G::at(R0 + r, C0 + c);
+      const T k0 = sf[at], k1 = sf[at + 1], k2 = sf[at + 2], k3 = sf[at + 3];
+      *reinterpret_cast<ulonglong2*>(v.keys + g) = make_ulonglong2(k0, k1);
+      *reinterpret_cast<ulonglong2*>(v.keys + g + 2) = make_ulonglong2(k2, k3);
+      *reinterpret_cast<int4*>(v.spill + g) = make_int4(spill_of(at, k0), spill_of(at + 1, k1),
+                                                       spill_of(at + 2, k2), spill_of(at + 3, k3));
+    }
+  } else {
+    for (int r = 0; r < 64; r++) {
+      const long long gx = qx + lane, gy = qy + r;
+      if (gx >= v.nx || gy >= v.ny) continue;
+      const size_t g = (size_t)gy * v.nx + gx;
+      const int at = G::at(R0 + r, C0 + lane);
+      const T val = sf[at];
+      v.keys[g] = val;
+      v.spill[g] = spill_of(at, val);
+    }
+  }
+}
+
 // Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
@@ -1069,6 +1106,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
       ch |= spill_sweep_regs<G, false, false>(x, sf, mrow, hs, hl, 1u, R0, C0, lane);
 #pragma unroll
       for (int c = 0; c < 64; c++) w[2 * G::at(R0 + lane, C0 + c)] = x[c];
+      wave_change |= __any(ch ? 1 : 0);
       any = __syncthreads_or(ch ? 1 : 0);
       any_change |= any;
       it++;
@@ -1090,7 +1128,11 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
   // input) take the input bits.
   if constexpr (MODE == kModeSpill) {
-    if (first || any_change) write_tile<G, MODE>(v, sf, sd, x0, y0);
+    // the first visit also seeded cells outside the sweeps: whole tile
+    if (first)
+      write_tile<G, MODE>(v, sf, sd, x0, y0);
+    else if (wave_change)
+      write_spill_quadrant<G>(v, sf, sd, x0, y0, R0, C0, lane);
   } else {
     // each wavefront writes its own quadrant if its sweeps changed it
     if (first || wave_change) write_quadrant<G, MODE>(v, sf, sd, x0, y0, R0, C0, lane);
