@@ -82,6 +82,40 @@ def test_fillsinks_shapes(cuda, oracle_lib, shape):
         assert_bitwise(out, want, f"shape {shape} tile {tile}")
 
 
+def test_results_without_statistics(cuda, oracle_lib):
+    """The default path collects no visit statistics (the visits skip their
+    counter atomics). Its fill and spill outputs are bit-identical to the
+    path with statistics and to the oracle; excess topography without
+    statistics stays within the FP64 tolerance."""
+    import libtopotoolbox_amd as tt
+
+    shape = (389, 517)  # partial tiles on both axes, nx % 4 != 0
+    rng = np.random.default_rng(23)
+    dem = oracle_lib.grid_random(7, shape)
+    dem[rng.uniform(size=shape) < 0.02] = np.nan
+    bc = edge_bc(shape)
+    want = oracle_lib.fillsinks_pf(dem, bc)
+    d = torch.from_numpy(np.ascontiguousarray(dem)).cuda()
+    b = torch.from_numpy(bc).cuda()
+    for tile in TILES:
+        tt.set_tile_shape(*tile)
+        off = tt.fillsinks(d, b).cpu().numpy()
+        assert_bitwise(off, want, f"fill without statistics, tile {tile}")
+        (lev, sp) = tt.ops.fillsinks_spill(d, b)
+        (lev_s, sp_s), st = tt.ops.fillsinks_spill(d, b, stats=True)
+        assert st["error"] == 0, st
+        assert_bitwise(lev.cpu().numpy(), want, f"spill levels, tile {tile}")
+        np.testing.assert_array_equal(sp.cpu().numpy(), sp_s.cpu().numpy(),
+                                      err_msg=f"spill cells with / without statistics, {tile}")
+    slopes = rng.uniform(0.05, 0.9, size=shape).astype(np.float32)
+    u64, _ = oracle_lib.fsm2d_f64(dem, slopes, 30.0)
+    s = torch.from_numpy(slopes).cuda()
+    for tile in TILES:
+        tt.set_tile_shape(*tile)
+        u = tt.excesstopography_fsm2d(d, s, 30.0).cpu().numpy()
+        fsm_close(u, u64, f"fsm without statistics, tile {tile}")
+
+
 def test_fillsinks_special_values(cuda, oracle_lib):
     rng = np.random.default_rng(5)
     dem = np.floor(rng.uniform(0, 8, size=(190, 170))).astype(np.float32)  # flats
