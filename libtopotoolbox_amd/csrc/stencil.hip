@@ -528,10 +528,102 @@ hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t
   return e;
 }
 
+// 3x3 structuring elements of one slice, streaming through registers:
+// each lane owns four adjacent columns of a kRows3-row strip and holds the
+// strip's rows plus the one above and below (16-byte loads, all in flight
+// together); the columns beside its four come from the neighbouring lanes
+// by DPP, the wavefront's outside columns by one extra load in its edge
+// lanes. Every cell folds the set entries in the reference's visiting
+// order (row, then column), as the tiled kernel does. Needs nx % 4 == 0
+// and 16-byte aligned arrays. Short strips (many wavefronts in flight),
+// nontemporal stores and per-XCD runs of strips measured fastest at
+// 8192^2: 0.096 ms against 0.170 ms for the tiled kernel.
+constexpr int kRows3 = 4;
+
+template <int OP>
+__device__ __forceinline__ void fold3x3_row(float (&acc)[4], const float (&w)[6], unsigned bits) {
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+      if (bits & (1u << dx)) acc[k] = fold<OP>(acc[k], w[k + dx]);
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void value_filter_3x3(float* out, const float* dem,
+                                                        const uint8_t* se, long long nx,
+                                                        long long ny) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // workgroups go round-robin over the 8 XCDs; when the grid divides by 8,
+  // renumber them so that each XCD takes a contiguous run of strips, whose
+  // shared halo rows then meet in its L2
+  const long long gx = gridDim.x, total = gx * gridDim.y;
+  long long L = (long long)blockIdx.y * gx + blockIdx.x;
+  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  const long long bx = L % gx, by = L / gx;
+  const long long x = (bx * 64 + lane) * 4;
+  const long long y0 = (by * 4 + wave) * kRows3;
+  if (y0 >= ny) return;  // whole wavefront
+  unsigned mask = 0u;  // bit 3 * row + column of the element
+#pragma unroll
+  for (int k = 0; k < 9; k++) mask |= (se[k] ? 1u : 0u) << k;
+  const float nan = __builtin_nanf("");
+  const bool xin = x < nx;
+  float4 c[kRows3 + 2];
+  float lft[kRows3 + 2], rgt[kRows3 + 2];
+#pragma unroll
+  for (int i = 0; i < kRows3 + 2; i++) {
+    const long long y = y0 - 1 + i;
+    const bool yin = y >= 0 && y < ny;
+    c[i] = make_float4(nan, nan, nan, nan);
+    lft[i] = nan;
+    rgt[i] = nan;
+    if (xin && yin) c[i] = *reinterpret_cast<const float4*>(dem + y * nx + x);
+    if (lane == 0 && yin && x >= 1 && x - 1 < nx) lft[i] = dem[y * nx + x - 1];
+    if (lane == 63 && yin && x + 4 < nx) rgt[i] = dem[y * nx + x + 4];
+  }
+  // row i's six columns x - 1 .. x + 4
+  float w[kRows3 + 2][6];
+#pragma unroll
+  for (int i = 0; i < kRows3 + 2; i++) {
+    w[i][0] = __int_as_float(__builtin_amdgcn_update_dpp(
+        __float_as_int(lft[i]), __float_as_int(c[i].w), 0x138, 0xf, 0xf, false));
+    w[i][1] = c[i].x;
+    w[i][2] = c[i].y;
+    w[i][3] = c[i].z;
+    w[i][4] = c[i].w;
+    w[i][5] = __int_as_float(__builtin_amdgcn_update_dpp(
+        __float_as_int(rgt[i]), __float_as_int(c[i].x), 0x130, 0xf, 0xf, false));
+  }
+  const float init = OP < 0 ? __builtin_inff() : -__builtin_inff();
+#pragma unroll
+  for (int j = 0; j < kRows3; j++) {
+    const long long y = y0 + j;
+    if (!xin || y >= ny) continue;
+    float acc[4] = {init, init, init, init};
+    fold3x3_row<OP>(acc, w[j], mask);
+    fold3x3_row<OP>(acc, w[j + 1], mask >> 3);
+    fold3x3_row<OP>(acc, w[j + 2], mask >> 6);
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = w[j + 1][k + 1] != w[j + 1][k + 1] ? nan : acc[k];
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f ov = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4f*>(out + y * nx + x));
+  }
+}
+
 template <int OP>
 static hipError_t value_filter_launch(float* out, const float* dem, const uint8_t* se,
                                       const int64_t d[3], int64_t nx, int64_t ny,
                                       hipStream_t stream) {
+  const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (d[0] == 3 && d[1] == 3 && d[2] == 1 && (nx & 3) == 0 && al(out) && al(dem) &&
+      (ny + 4 * kRows3 - 1) / (4 * kRows3) <= 65535) {
+    const dim3 g((unsigned)((nx + 255) / 256), (unsigned)((ny + 4 * kRows3 - 1) / (4 * kRows3)));
+    value_filter_3x3<OP><<<g, 256, 0, stream>>>(out, dem, se, nx, ny);
+    return hipGetLastError();
+  }
   const bool tiled = d[0] >= 1 && d[1] >= 1 && d[2] >= 1 && d[0] <= 2 * kHaloMax + 1 &&
                      d[1] <= 2 * kHaloMax + 1 && d[0] * d[1] * d[2] <= kSeMax;
   if (tiled) {
