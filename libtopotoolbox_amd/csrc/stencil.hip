@@ -233,6 +233,142 @@ __global__ __launch_bounds__(kThreads, 5) void identifyflats_kernel(
   }
 }
 
+// identifyflats streaming through registers (nx % 4 == 0, 16-byte aligned
+// arrays), laid out as value_filter_3x3: each lane owns four columns of a
+// kRowsF-row strip. It holds the values of rows -2 .. kRowsF + 1 and
+// columns -2 .. 5 around them (side columns by DPP, the wavefront's
+// outside ones by a load in its edge lanes), computes the flat bits of
+// rows -1 .. kRowsF and columns -1 .. 4 itself, then the flags of its own
+// cells, with the tiled kernel's rules.
+constexpr int kRowsF = 4;
+
+__global__ __launch_bounds__(256) void identifyflats_stream(int32_t* out, const float* dem,
+                                                            long long nx, long long ny,
+                                                            unsigned long long* partial) {
+  __shared__ unsigned long long wave_count[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // per-XCD runs of strips, as in value_filter_3x3
+  const long long gxb = gridDim.x, total = gxb * gridDim.y;
+  long long L = (long long)blockIdx.y * gxb + blockIdx.x;
+  const long long block_id = L;
+  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  const long long bx = L % gxb, by = L / gxb;
+  const long long x = (bx * 64 + lane) * 4;
+  const long long y0 = (by * 4 + wave) * kRowsF;
+  const bool active = y0 < ny, xin = x < nx;
+  const float nan = __builtin_nanf("");
+  constexpr int NR = kRowsF + 4;
+  float v[NR][8];  // value rows y0 - 2 + i, columns x - 2 + j
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    const long long y = y0 - 2 + i;
+    const bool yin = active && y >= 0 && y < ny;
+    float4 c = make_float4(nan, nan, nan, nan);
+    float2 lw = make_float2(nan, nan), rw = make_float2(nan, nan);
+    if (xin && yin) c = *reinterpret_cast<const float4*>(dem + y * nx + x);
+    if (lane == 0 && yin && x >= 2 && x - 2 < nx)
+      lw = *reinterpret_cast<const float2*>(dem + y * nx + x - 2);
+    if (lane == 63 && yin && x + 4 < nx)
+      rw = *reinterpret_cast<const float2*>(dem + y * nx + x + 4);
+    auto shr = [](float old, float src) {  // lane i <- lane i - 1
+      return __int_as_float(
+          __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x138, 0xf, 0xf, false));
+    };
+    auto shl = [](float old, float src) {  // lane i <- lane i + 1
+      return __int_as_float(
+          __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf, false));
+    };
+    v[i][0] = shr(lw.x, c.z);
+    v[i][1] = shr(lw.y, c.w);
+    v[i][2] = c.x;
+    v[i][3] = c.y;
+    v[i][4] = c.z;
+    v[i][5] = c.w;
+    v[i][6] = shl(rw.x, c.x);
+    v[i][7] = shl(rw.y, c.y);
+  }
+  // flat bits: fl[fr] bit fc is the cell (y0 - 1 + fr, x - 1 + fc), an
+  // interior cell equal to the minimum of its 3x3 window with NaN
+  // neighbours counting as -INF. Compared as order-preserving integer keys
+  // (NaN as -INF, -0 as +0, since the float comparison finds them equal),
+  // a non-NaN cell is flat exactly when the 3x3 minimum key (the cell
+  // included) is its own; the minimum is taken over columns, then rows.
+  unsigned kk[NR][8];
+#pragma unroll
+  for (int i = 0; i < NR; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float q = v[i][j];
+      const unsigned u = __float_as_uint(q + 0.0f);  // -0 + 0 = +0
+      kk[i][j] = q != q ? 0x007fffffu : (u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u));
+    }
+  // interior columns of the six flat columns (bit fc: x - 1 + fc)
+  unsigned colin = 0u;
+#pragma unroll
+  for (int fc = 0; fc < 6; fc++) colin |= (unsigned)(x - 1 + fc > 0 && x - 1 + fc < nx - 1) << fc;
+  unsigned fl[kRowsF + 2];
+#pragma unroll
+  for (int fr = 0; fr < kRowsF + 2; fr++) {
+    const long long gy = y0 - 1 + fr;
+    unsigned cm[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) cm[j] = min(min(kk[fr][j], kk[fr + 1][j]), kk[fr + 2][j]);
+    unsigned bits = 0u;
+#pragma unroll
+    for (int fc = 0; fc < 6; fc++) {
+      const float h = v[fr + 1][fc + 1];
+      const unsigned m9 = min(min(cm[fc], cm[fc + 1]), cm[fc + 2]);
+      bits |= (unsigned)(h == h && m9 == kk[fr + 1][fc + 1]) << fc;
+    }
+    fl[fr] = (gy > 0 && gy < ny - 1) ? (bits & colin) : 0u;
+  }
+  // flags of the own cells. A neighbour outside the grid is NaN in the
+  // window, never equal to anything, so it needs no test of its own.
+  // Neighbour n of a cell (n = 0..7: rows -1, 0, +1, columns -1, 0, +1 in
+  // order, the cell itself skipped): eq bit n if its value equals the
+  // cell's, nf bit n if it is flat.
+  unsigned long long flats = 0;
+#pragma unroll
+  for (int r = 0; r < kRowsF; r++) {
+    const long long gy = y0 + r;
+    if (!active || !xin || gy >= ny) continue;
+    int o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float h = v[r + 2][k + 2];
+      const unsigned flat = (fl[r + 1] >> (k + 1)) & 1u;
+      const unsigned nf = ((fl[r] >> k) & 7u) | (((fl[r + 1] >> k) & 1u) << 3) |
+                          (((fl[r + 1] >> (k + 2)) & 1u) << 4) | (((fl[r + 2] >> k) & 7u) << 5);
+      unsigned eq = 0u;
+      int n = 0;
+#pragma unroll
+      for (int dr = -1; dr <= 1; dr++)
+#pragma unroll
+        for (int dc = -1; dc <= 1; dc++) {
+          if (dr == 0 && dc == 0) continue;
+          eq |= (unsigned)(v[r + 2 + dr][k + 2 + dc] == h) << n;
+          n++;
+        }
+      // sill: not flat, touches an equal flat; presill: flat, touches an
+      // equal non-flat cell (a sill)
+      const unsigned sill = !flat && (eq & nf) != 0u;
+      const unsigned presill = flat && (eq & ~nf) != 0u;
+      o[k] = (int)(flat | (sill << 1) | (presill << 2));
+    }
+    flats += __popc((fl[r + 1] >> 1) & 15u);
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i ov = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4i*>(out + gy * nx + x));
+  }
+  if (partial) {
+    for (int off = 32; off > 0; off >>= 1) flats += __shfl_xor(flats, off);
+    if (lane == 0) wave_count[wave] = flats;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      partial[block_id] = wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+  }
+}
+
 __global__ void sum_partials(unsigned long long* partial, int n) {
   __shared__ unsigned long long s[256];
   unsigned long long v = 0;
@@ -502,15 +638,27 @@ hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t
     if (count_host) *count_host = 0;
     return hipSuccess;
   }
+  const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const long long sgx = (nx + 255) / 256, sgy = (ny + 4 * kRowsF - 1) / (4 * kRowsF);
+  const bool stream_path = (nx & 3) == 0 && al(out) && al(dem) && sgy <= 65535;
   int grid = 1;
-  hipError_t e = tiled_grid((const void*)identifyflats_kernel, 0, tile_count(nx, ny), &grid);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (stream_path) {
+    grid = (int)(sgx * sgy);
+  } else {
+    e = tiled_grid((const void*)identifyflats_kernel, 0, tile_count(nx, ny), &grid);
+    if (e != hipSuccess) return e;
+  }
   unsigned long long* partial = nullptr;
   if (count_host) {
     e = hipMallocAsync((void**)&partial, sizeof(*partial) * (grid + 1), stream);
     if (e != hipSuccess) return e;
   }
-  identifyflats_kernel<<<grid, kThreads, 0, stream>>>(out, dem, nx, ny, partial);
+  if (stream_path)
+    identifyflats_stream<<<dim3((unsigned)sgx, (unsigned)sgy), 256, 0, stream>>>(out, dem, nx, ny,
+                                                                                 partial);
+  else
+    identifyflats_kernel<<<grid, kThreads, 0, stream>>>(out, dem, nx, ny, partial);
   e = hipGetLastError();
   if (count_host) {
     if (e == hipSuccess) {

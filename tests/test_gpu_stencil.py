@@ -11,7 +11,8 @@ from conftest import assert_bitwise
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(1, 1), (1, 37), (37, 1), (2, 3), (3, 3), (8, 64), (9, 65), (57, 83), (130, 513)]
+SHAPES = [(1, 1), (1, 37), (37, 1), (2, 3), (3, 3), (8, 64), (9, 65), (57, 83), (130, 513),
+          (1, 4), (4, 4), (67, 260), (130, 1028)]  # nx % 4 == 0: the streaming kernel
 
 
 def stencil_dem(rng, shape, flats=True):
