@@ -802,9 +802,106 @@ hipError_t run_value_filter(int op, float* out, const float* dem, const uint8_t*
                 : value_filter_launch<1>(out, dem, se, se_dims, nx, ny, stream);
 }
 
+// Square elements of width 1..5 streaming through registers (nx % 4 ==
+// 0, 16-byte aligned arrays), laid out as identifyflats_stream: each lane
+// owns four columns of a kRowsS-row strip and holds the rows its window
+// needs, columns -2 .. 5 around its own. Each row is folded along x
+// (written to tmp for the strip's own rows; rows outside the grid become
+// NaN, which the vertical fold skips), then each cell along y, both in the
+// reference's order; a NaN input cell gives NaN.
+constexpr int kRowsS = 4;
+
+template <int OP, int WD>
+__global__ __launch_bounds__(256) void value_filter_square_stream(float* out, float* tmp,
+                                                                  const float* dem, long long nx,
+                                                                  long long ny) {
+  constexpr int C = (WD + 1) / 2 - 1;  // cells before the centre on each axis
+  constexpr int NR = kRowsS + WD - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // per-XCD runs of strips, as in value_filter_3x3
+  const long long gxb = gridDim.x, total = gxb * gridDim.y;
+  long long L = (long long)blockIdx.y * gxb + blockIdx.x;
+  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  const long long bx = L % gxb, by = L / gxb;
+  const long long x = (bx * 64 + lane) * 4;
+  const long long y0 = (by * 4 + wave) * kRowsS;
+  if (y0 >= ny) return;  // whole wavefront
+  const bool xin = x < nx;
+  const float nan = __builtin_nanf("");
+  const float init = OP < 0 ? __builtin_inff() : -__builtin_inff();
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  float h[NR][4];          // row y0 - C + i folded along x
+  unsigned self_nan[kRowsS];  // bit k: input cell (y0 + r, x + k) is NaN
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    const long long y = y0 - C + i;
+    const bool yin = y >= 0 && y < ny;
+    float4 c = make_float4(nan, nan, nan, nan);
+    float2 lw = make_float2(nan, nan), rw = make_float2(nan, nan);
+    if (xin && yin) c = *reinterpret_cast<const float4*>(dem + y * nx + x);
+    if (lane == 0 && yin && x >= 2 && x - 2 < nx)
+      lw = *reinterpret_cast<const float2*>(dem + y * nx + x - 2);
+    if (lane == 63 && yin && x + 4 < nx)
+      rw = *reinterpret_cast<const float2*>(dem + y * nx + x + 4);
+    const float w[8] = {  // columns x - 2 .. x + 5; lane i - 1 / i + 1 by DPP
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lw.x), __float_as_int(c.z),
+                                                   0x138, 0xf, 0xf, false)),
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lw.y), __float_as_int(c.w),
+                                                   0x138, 0xf, 0xf, false)),
+        c.x, c.y, c.z, c.w,
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(rw.x), __float_as_int(c.x),
+                                                   0x130, 0xf, 0xf, false)),
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(rw.y), __float_as_int(c.y),
+                                                   0x130, 0xf, 0xf, false))};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float acc = init;
+#pragma unroll
+      for (int a = 0; a < WD; a++) acc = fold<OP>(acc, w[k - C + a + 2]);
+      h[i][k] = yin ? acc : nan;
+    }
+    if (i >= C && i < C + kRowsS) {
+      self_nan[i - C] = (unsigned)(c.x != c.x) | ((unsigned)(c.y != c.y) << 1) |
+                        ((unsigned)(c.z != c.z) << 2) | ((unsigned)(c.w != c.w) << 3);
+      if (xin && yin) {
+        const v4f tv = {h[i][0], h[i][1], h[i][2], h[i][3]};
+        __builtin_nontemporal_store(tv, reinterpret_cast<v4f*>(tmp + y * nx + x));
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRowsS; r++) {
+    const long long y = y0 + r;
+    if (!xin || y >= ny) continue;
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float acc = init;
+#pragma unroll
+      for (int a = 0; a < WD; a++) acc = fold<OP>(acc, h[r + a][k]);
+      o[k] = ((self_nan[r] >> k) & 1u) ? nan : acc;
+    }
+    const v4f ov = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(ov, reinterpret_cast<v4f*>(out + y * nx + x));
+  }
+}
+
 template <int OP>
 static hipError_t square_launch(float* out, const float* dem, float* tmp, int width, int64_t nx,
                                 int64_t ny, hipStream_t stream) {
+  const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (width >= 1 && width <= 5 && (nx & 3) == 0 && al(out) && al(tmp) && al(dem) &&
+      (ny + 4 * kRowsS - 1) / (4 * kRowsS) <= 65535) {
+    const dim3 g((unsigned)((nx + 255) / 256), (unsigned)((ny + 4 * kRowsS - 1) / (4 * kRowsS)));
+    switch (width) {
+      case 1: value_filter_square_stream<OP, 1><<<g, 256, 0, stream>>>(out, tmp, dem, nx, ny); break;
+      case 2: value_filter_square_stream<OP, 2><<<g, 256, 0, stream>>>(out, tmp, dem, nx, ny); break;
+      case 3: value_filter_square_stream<OP, 3><<<g, 256, 0, stream>>>(out, tmp, dem, nx, ny); break;
+      case 4: value_filter_square_stream<OP, 4><<<g, 256, 0, stream>>>(out, tmp, dem, nx, ny); break;
+      default: value_filter_square_stream<OP, 5><<<g, 256, 0, stream>>>(out, tmp, dem, nx, ny); break;
+    }
+    return hipGetLastError();
+  }
   if (width >= 1 && width <= kSquareMax) {
     const size_t lds = sizeof(float) * (size_t)(kTH + width - 1) * (size_t)(kTW + width - 1);
     const int rows = kTH + width - 1;

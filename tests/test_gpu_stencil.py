@@ -109,7 +109,9 @@ def test_value_filter_square(oracle_lib, width, op):
 
     fn = tt.min_filter_square if op == "min" else tt.max_filter_square
     rng = np.random.default_rng(300 + width)
-    for shape in [(1, 1), (3, 2), (1, 300), (300, 1), (57, 83), (260, 129)]:
+    # nx % 4 == 0 shapes take the streaming kernel for widths 1..5
+    for shape in [(1, 1), (3, 2), (1, 300), (300, 1), (57, 83), (260, 129), (2, 4), (67, 260),
+                  (130, 1028)]:
         dem = stencil_dem(rng, shape)
         want = oracle_lib.value_filter_square(dem, width, op)
         tmp = torch.empty(shape, dtype=torch.float32, device="cuda")
