@@ -128,57 +128,6 @@ struct Ws {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.nx; \
          i += (long long)gridDim.x * blockDim.x)
 
-// Same pixels, but every lane of a wave runs every iteration (lanes past
-// the row's end with in_row false), for kernels that combine across lanes.
-#define GW_FOR_PIXELS_WAVE                                                         \
-  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y)                         \
-    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < g.nx;             \
-         i0 += (long long)gridDim.x * blockDim.x)                                  \
-      for (long long i = i0 + threadIdx.x, once = 1; once; once = 0)               \
-        if (const bool in_row = i < g.nx; true)
-
-// Combining atomics over a wave: the lanes holding the same key (kNone:
-// no contribution) reduce their values and one of them applies the
-// atomics, once per distinct key in the wave. Must be called by all lanes.
-__device__ void wave_or_max(unsigned* or_to, unsigned* max_to, unsigned key, unsigned orv,
-                            unsigned maxv) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long pending = __ballot(key != kNone);
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const unsigned k0 = __shfl(key, leader);
-    const bool mine = key == k0;
-    unsigned o = mine ? orv : 0u, m = mine ? maxv : 0u;
-    for (int d = 32; d > 0; d >>= 1) {
-      o |= __shfl_xor(o, d);
-      m = max(m, __shfl_xor(m, d));
-    }
-    if (lane == leader) {
-      if (o) atomicOr(or_to + k0, o);
-      if (m) atomicMax(max_to + k0, m);
-    }
-    pending &= ~__ballot(mine);
-  }
-}
-
-__device__ void wave_count_min(unsigned* count_to, unsigned* min_to, unsigned key,
-                               unsigned value) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long pending = __ballot(key != kNone);
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const unsigned k0 = __shfl(key, leader);
-    const unsigned long long group = __ballot(key == k0);
-    unsigned m = key == k0 ? value : kNone;
-    for (int d = 32; d > 0; d >>= 1) m = min(m, __shfl_xor(m, d));
-    if (lane == leader) {
-      if (count_to) atomicAdd(count_to + k0, (unsigned)__popcll(group));
-      atomicMin(min_to + k0, m);
-    }
-    pending &= ~group;
-  }
-}
-
 // One flat scanned neighbour of (i, j), or the pixel itself if it has none
 // (a birth). The trees these links form are the chain groups; their
 // smallest pixel, the root, is their birth.
@@ -299,37 +248,138 @@ __device__ __forceinline__ float depth_at(const float* original, const float* fi
   return filled[p] - original[p];
 }
 
-__global__ void gw_depth_max(Grid g, Ws w, const float* original, const float* filled) {
-  GW_FOR_PIXELS_WAVE {
-    const long long p = j * g.nx + i;
-    unsigned c = kNone, kind = 0u, wb = 0u;
-    if (in_row && (g.flats[p] & 1)) {
-      const float d = depth_at(original, filled, p);
-      c = w.C[w.A[p]];
-      kind = d != d ? kHasNaN : kHasNumber;
-      wb = d != d ? 0u : gw_ord(d);
-    }
-    wave_or_max(w.K, w.W, c, kind, wb);
-  }
-}
-
 // Whether p is one of the deepest pixels of its component of kind k.
 __device__ __forceinline__ bool deepest_pixel(unsigned k, float d, unsigned wbits) {
   return k == kHasNaN ? d != d : d == gw_unord(wbits);  // float ==: -0 and +0 tie
 }
 
-__global__ void gw_deepest(Grid g, Ws w, const float* original, const float* filled) {
-  GW_FOR_PIXELS_WAVE {
-    const long long p = j * g.nx + i;
-    unsigned c = kNone, b = kNone;
-    if (in_row && (g.flats[p] & 1)) {
-      b = w.A[p];
-      const unsigned cc = w.C[b], k = w.K[cc];
-      if (k != kMixed && deepest_pixel(k, depth_at(original, filled, p), w.W[cc])) c = cc;
-      else b = kNone;
+// Per component: kind bits and largest depth (gw_depth_max_runs); the
+// number of deepest pixels and the first of them, and per birth the first
+// deepest pixel of its chain group (gw_deepest_runs). One wavefront per
+// block walks kRunChunks consecutive 64-pixel chunks of a row and keeps a
+// running entry (wave-uniform) per target array, flushed with the atomics
+// only when the key changes. Flats come in long runs of one component, so
+// a lake's waves no longer all hit its words once per 64 pixels; the
+// atomics are associative and commutative, so the words end the same.
+constexpr int kRunChunks = 16;
+
+__global__ __launch_bounds__(64) void gw_depth_max_runs(Grid g, Ws w, const float* original,
+                                                        const float* filled) {
+  const int lane = threadIdx.x;
+  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y) {
+    unsigned ck = kNone, co = 0u, cm = 0u;
+    auto flush = [&]() {
+      if (ck != kNone && lane == 0) {
+        if (co) atomicOr(w.K + ck, co);
+        if (cm) atomicMax(w.W + ck, cm);
+      }
+    };
+    const long long ibase = (long long)blockIdx.x * 64 * kRunChunks;
+    for (int t = 0; t < kRunChunks; t++) {
+      const long long i0 = ibase + 64LL * t;
+      if (i0 >= g.nx) break;
+      const long long i = i0 + lane;
+      const long long p = j * g.nx + i;
+      unsigned c = kNone, kind = 0u, wb = 0u;
+      if (i < g.nx && (g.flats[p] & 1)) {
+        const float d = depth_at(original, filled, p);
+        c = w.C[w.A[p]];
+        kind = d != d ? kHasNaN : kHasNumber;
+        wb = d != d ? 0u : gw_ord(d);
+      }
+      unsigned long long pending = __ballot(c != kNone);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned k0 = __shfl(c, leader);
+        const bool mine = c == k0;
+        unsigned o = mine ? kind : 0u, m = mine ? wb : 0u;
+        for (int dd = 32; dd > 0; dd >>= 1) {
+          o |= __shfl_xor(o, dd);
+          m = max(m, __shfl_xor(m, dd));
+        }
+        if (k0 == ck) {
+          co |= o;
+          cm = max(cm, m);
+        } else {
+          flush();
+          ck = k0;
+          co = o;
+          cm = m;
+        }
+        pending &= ~__ballot(mine);
+      }
     }
-    wave_count_min(w.Mc, w.Mf, c, (unsigned)p);
-    wave_count_min(nullptr, w.D, b, (unsigned)p);
+    flush();
+  }
+}
+
+__global__ __launch_bounds__(64) void gw_deepest_runs(Grid g, Ws w, const float* original,
+                                                      const float* filled) {
+  const int lane = threadIdx.x;
+  for (long long j = blockIdx.y; j < g.ny; j += gridDim.y) {
+    // running entries: component (count of deepest pixels, first one) and
+    // birth (first deepest pixel of its chain group)
+    unsigned ck = kNone, ccount = 0u, cmin = kNone, bk = kNone, bmin = kNone;
+    auto flush_c = [&]() {
+      if (ck != kNone && lane == 0) {
+        atomicAdd(w.Mc + ck, ccount);
+        atomicMin(w.Mf + ck, cmin);
+      }
+    };
+    auto flush_b = [&]() {
+      if (bk != kNone && lane == 0) atomicMin(w.D + bk, bmin);
+    };
+    const long long ibase = (long long)blockIdx.x * 64 * kRunChunks;
+    for (int t = 0; t < kRunChunks; t++) {
+      const long long i0 = ibase + 64LL * t;
+      if (i0 >= g.nx) break;
+      const long long i = i0 + lane;
+      const long long p = j * g.nx + i;
+      unsigned c = kNone, b = kNone;
+      if (i < g.nx && (g.flats[p] & 1)) {
+        b = w.A[p];
+        const unsigned cc = w.C[b], k = w.K[cc];
+        if (k != kMixed && deepest_pixel(k, depth_at(original, filled, p), w.W[cc])) c = cc;
+        else b = kNone;
+      }
+      unsigned long long pending = __ballot(c != kNone);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned k0 = __shfl(c, leader);
+        const unsigned long long group = __ballot(c == k0);
+        unsigned m = c == k0 ? (unsigned)p : kNone;
+        for (int dd = 32; dd > 0; dd >>= 1) m = min(m, __shfl_xor(m, dd));
+        const unsigned n = (unsigned)__popcll(group);
+        if (k0 == ck) {
+          ccount += n;
+          cmin = min(cmin, m);
+        } else {
+          flush_c();
+          ck = k0;
+          ccount = n;
+          cmin = m;
+        }
+        pending &= ~group;
+      }
+      pending = __ballot(b != kNone);
+      while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned k0 = __shfl(b, leader);
+        const unsigned long long group = __ballot(b == k0);
+        unsigned m = b == k0 ? (unsigned)p : kNone;
+        for (int dd = 32; dd > 0; dd >>= 1) m = min(m, __shfl_xor(m, dd));
+        if (k0 == bk) {
+          bmin = min(bmin, m);
+        } else {
+          flush_b();
+          bk = k0;
+          bmin = m;
+        }
+        pending &= ~group;
+      }
+    }
+    flush_c();
+    flush_b();
   }
 }
 
@@ -515,8 +565,12 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   gw_components<<<grid, 256, 0, s>>>(g, w);
   gw_component_roots<<<grid, 256, 0, s>>>(g, w);
   gw_component_store<<<grid, 256, 0, s>>>(g, w);
-  gw_depth_max<<<grid, 256, 0, s>>>(g, w, original, filled);
-  gw_deepest<<<grid, 256, 0, s>>>(g, w, original, filled);
+  {
+    const long long span = 64LL * kRunChunks;
+    const dim3 rgrid((unsigned)((nx + span - 1) / span), (unsigned)(ny < 65535 ? ny : 65535));
+    gw_depth_max_runs<<<rgrid, 64, 0, s>>>(g, w, original, filled);
+    gw_deepest_runs<<<rgrid, 64, 0, s>>>(g, w, original, filled);
+  }
   unsigned* d_count = w.err + 1;
   gw_events<<<grid, 256, 0, s>>>(g, w, nullptr, d_count);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
