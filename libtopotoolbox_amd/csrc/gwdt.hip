@@ -203,17 +203,21 @@ __global__ void gw_link(Grid g, Ws w) {
 // a root stored into A[p] itself could be overwritten by another thread's
 // path halving that read A[p]'s old parent just before, leaving a
 // non-root behind.
+// Only run starts are nodes of the union-find: every other flat points at
+// a run start (its own, or an ancestor after path halving in gw_link) and
+// is never on anyone's path, so the roots are found from the run starts
+// alone and the other flats take their run start's root.
 __global__ void gw_births(Grid g, Ws w) {
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
-    if (g.flats[p] & 1) w.F[p] = find_halving(w.A, (unsigned)p);
+    if ((g.flats[p] & 1) && !g.flat(i - 1, j)) w.F[p] = find_halving(w.A, (unsigned)p);
   }
 }
 
 __global__ void gw_births_store(Grid g, Ws w) {
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
-    if (g.flats[p] & 1) w.A[p] = w.F[p];
+    if (g.flats[p] & 1) w.A[p] = w.F[g.flat(i - 1, j) ? w.A[p] : (unsigned)p];
   }
 }
 
