@@ -27,12 +27,13 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
 }
 
 template <class G, int MODE>
-static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats) {
+static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats,
+                              void* scratch = nullptr) {
   a.ntx = (int)((a.v.nx + G::TX - 1) / G::TX);
   a.nty = (int)((a.v.ny + G::TY - 1) / G::TY);
   a.max_local = 1u << 16;
   return launch_persistent(fill_persistent<G, MODE>, G::THREADS, a.ntx * a.nty, a, stream, stats,
-                           G::TX, G::TY, a.ntx, a.nty);
+                           G::TX, G::TY, a.ntx, a.nty, scratch);
 }
 
 template <int MODE>

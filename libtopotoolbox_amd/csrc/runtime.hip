@@ -70,13 +70,21 @@ __global__ void init_queue(Sync* s, unsigned long long* ring, unsigned* state,
 hipError_t Workspace::alloc(int ntiles, hipStream_t s) { return alloc(ntiles, 0, 0, s); }
 
 hipError_t Workspace::alloc(int ntiles, int ntx, int nty, hipStream_t s) {
+  void* mem = nullptr;
+  hipError_t e = hipMallocAsync(&mem, queue_workspace_bytes(ntiles), s);
+  if (e != hipSuccess) return e;
+  e = attach(mem, ntiles, ntx, nty, s);
+  owned = true;
+  return e;
+}
+
+hipError_t Workspace::attach(void* mem, int ntiles, int ntx, int nty, hipStream_t s) {
   stream = s;
+  base = mem;
+  owned = false;
   // Sync block (rounded to 64 words) + ring (2 words per tile) + state and
   // visited (1 word per tile)
   const size_t head = 64;
-  const size_t words = head + (size_t)ntiles * 4;
-  hipError_t e = hipMallocAsync(&base, words * sizeof(unsigned), s);
-  if (e != hipSuccess) return e;
   q.sync = reinterpret_cast<Sync*>(base);
   q.ring = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned*>(base) + head);
   q.state = reinterpret_cast<unsigned*>(q.ring + ntiles);
@@ -85,12 +93,6 @@ hipError_t Workspace::alloc(int ntiles, int ntx, int nty, hipStream_t s) {
   int blocks = (ntiles + 255) / 256;
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
-  // TT_QUEUE_ROWMAJOR=1 keeps index order (diagnostics: compares orders)
-  static const bool rowmajor = [] {
-    const char* v = getenv("TT_QUEUE_ROWMAJOR");
-    return v && v[0] == '1';
-  }();
-  if (rowmajor) ntx = nty = 0;
   init_queue<<<blocks, 256, 0, s>>>(q.sync, q.ring, q.state, q.visited, ntiles, ntx, nty);
   return hipGetLastError();
 }
@@ -113,8 +115,9 @@ hipError_t Workspace::finish(RunStats* st) {
       st->t_wait = (int64_t)h.t_wait;
     }
   }
-  hipError_t f = hipFreeAsync(base, stream);
+  hipError_t f = owned ? hipFreeAsync(base, stream) : hipSuccess;
   base = nullptr;
+  owned = false;
   return e != hipSuccess ? e : f;
 }
 

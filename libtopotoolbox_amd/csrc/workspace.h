@@ -1,7 +1,8 @@
 // Per-launch device workspace of the persistent relaxation kernels: the
 // control block and the tile queue (ring, per-tile state and visited flags).
-// Allocated stream-ordered (hipMallocAsync) and initialised by a kernel, so
-// a launch sequence contains no memset node.
+// Allocated stream-ordered (hipMallocAsync), or laid out in memory the
+// caller provides, and initialised by a kernel, so a launch sequence
+// contains no memset node.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,8 +14,14 @@ namespace tt {
 
 __global__ void zero_words(unsigned* p, size_t n);
 
+// Bytes of the queue workspace of ntiles tiles.
+inline size_t queue_workspace_bytes(int ntiles) {
+  return (64 + (size_t)ntiles * 4) * sizeof(unsigned);
+}
+
 struct Workspace {
   void* base = nullptr;
+  bool owned = false;  // base was allocated here (released by finish)
   TileQueue q{};
   hipStream_t stream = nullptr;
 
@@ -22,27 +29,33 @@ struct Workspace {
   hipError_t alloc(int ntiles, hipStream_t s);
   // All tiles of an ntx x nty grid start queued, boundary rings first.
   hipError_t alloc(int ntiles, int ntx, int nty, hipStream_t s);
+  // The same layout in caller memory `mem` (device, 256-byte aligned, at
+  // least queue_workspace_bytes(ntiles)); nothing is allocated or freed.
+  hipError_t attach(void* mem, int ntiles, int ntx, int nty, hipStream_t s);
   // Copies the control block into `st` (synchronises the stream) when st
-  // is non-null, then releases the memory stream-ordered.
+  // is non-null, then releases allocated memory stream-ordered.
   hipError_t finish(RunStats* st);
 };
 
 // Workgroups of the persistent grid: co-resident blocks, capped by tiles.
 hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* grid);
 
-// Allocates the workspace, launches `kern` as a persistent grid over
-// `ntiles` tiles (Args must carry a TileQueue `q`) and releases the
-// workspace stream-ordered; fills `stats` (synchronising) when non-null.
-// Tiles of an ntx x nty grid start in boundary-first order.
+// Allocates the workspace (or lays it out in `scratch`, device memory of
+// queue_workspace_bytes(ntiles), when non-null), launches `kern` as a
+// persistent grid over `ntiles` tiles (Args must carry a TileQueue `q`)
+// and releases an allocated workspace stream-ordered; fills `stats`
+// (synchronising) when non-null. Tiles of an ntx x nty grid start in
+// boundary-first order.
 template <class Args>
 hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a,
                              hipStream_t stream, RunStats* stats, int tile_x, int tile_y,
-                             int ntx = 0, int nty = 0) {
+                             int ntx = 0, int nty = 0, void* scratch = nullptr) {
   int grid = 0;
   hipError_t e = persistent_grid((const void*)kern, threads, ntiles, &grid);
   if (e != hipSuccess) return e;
   Workspace ws;
-  e = ws.alloc(ntiles, ntx, nty, stream);
+  e = scratch ? ws.attach(scratch, ntiles, ntx, nty, stream)
+              : ws.alloc(ntiles, ntx, nty, stream);
   if (e != hipSuccess) return e;
   a.q = ws.q;
   a.q.stats = stats ? 1 : 0;
