@@ -144,6 +144,18 @@ typedef struct tt_stats {
 int tt_fillsinks_dev(float *output, const float *dem, const uint8_t *bc, ptrdiff_t nx,
                      ptrdiff_t ny, void *stream, tt_stats *stats);
 
+/* fillsinks on device buffers that allocates nothing. No counterpart in
+   the reference: for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). Same result as tt_fillsinks_dev, bit for bit. The work-queue
+   state lives in a fixed 4 MiB array per GPU, held until the process ends,
+   which limits the grid to 262128 tiles of 128 x 128 cells (about 4.29e9
+   cells); larger grids are rejected with status 2. Every call on a GPU
+   shares that GPU's array: calls that overlap in time on one GPU race and
+   must be serialised by the caller (one stream, or events). */
+int tt_fillsinks_noalloc_dev(float *output, const float *dem, const uint8_t *bc, ptrdiff_t nx,
+                             ptrdiff_t ny, void *stream);
+
 /* reconstruct on device buffers; `marker` is updated in place. */
 int tt_reconstruct_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t ny,
                        void *stream, tt_stats *stats);

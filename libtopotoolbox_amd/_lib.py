@@ -92,6 +92,7 @@ SIGNATURES = {
     "min_filter_square": (None, [_vp, _vp, _vp, ctypes.c_uint8, _dims_t]),
     "max_filter_square": (None, [_vp, _vp, _vp, ctypes.c_uint8, _dims_t]),
     "tt_fillsinks_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_fillsinks_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp]),
     "tt_reconstruct_dev": (ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_excesstopography_fsm2d_dev": (
         ctypes.c_int, [_vp, _vp, _vp, ctypes.c_float, _pd, _pd, _vp, ctypes.POINTER(Stats)]),

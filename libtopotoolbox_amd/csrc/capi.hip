@@ -128,6 +128,26 @@ int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff
   return 0;
 }
 
+int tt_fillsinks_noalloc_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff_t nx,
+                             ptrdiff_t ny, void* stream) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("tt_fillsinks_noalloc_dev: invalid dims");
+    return 1;
+  }
+  if (tt::noalloc_fill_tiles(nx, ny) > tt::kNoAllocMaxTiles) {
+    tt::set_error(
+        "tt_fillsinks_noalloc_dev: %lld x %lld cells need %lld tiles of 128 x 128; the fixed "
+        "workspace holds at most %lld (use tt_fillsinks_dev)",
+        (long long)nx, (long long)ny, (long long)tt::noalloc_fill_tiles(nx, ny),
+        (long long)tt::kNoAllocMaxTiles);
+    return 2;
+  }
+  hipError_t e = tt::run_fill_noalloc(output, dem, bc, nx, ny, (hipStream_t)stream);
+  if (e != hipSuccess) return fail("tt_fillsinks_noalloc_dev", e);
+  return 0;
+}
+
 int tt_reconstruct_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t ny,
                        void* stream, tt_stats* stats) {
   tt::set_error("%s", "");

@@ -1,5 +1,7 @@
 // Sink filling and grayscale reconstruction on one grid: the persistent
 // work-queue kernel over fill_visit (fill_visit.h states the semantics).
+#include <mutex>
+
 #include "fill_visit.h"
 #include "workspace.h"
 
@@ -100,6 +102,63 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   e = dispatch_fill<kModeSpill>(a, ts, stream, stats);
   hipError_t f = hipFreeAsync(a.v.keys, stream);
   return e != hipSuccess ? e : f;
+}
+
+// ---------------------------------------------------------------- no allocation
+// fillsinks for callers that cannot allocate device memory (code under HIP
+// graph capture, tight serving loops): the queue workspace lives in a
+// fixed file-scope array instead of a stream-ordered allocation. Each GPU
+// holds its own copy of the array until the process ends, and every call
+// on that GPU shares it, so calls that overlap in time on one GPU race.
+// The init kernel rewrites the whole control block and queue each call, so
+// nothing depends on what the previous call left there.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per GPU
+}  // namespace
+
+// Device address of this GPU's `scratch` (the array's name in host code is
+// only a stand-in for it), looked up once per device with that device
+// current, on the device's first call.
+static hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+int64_t noalloc_fill_tiles(int64_t nx, int64_t ny) {
+  using G = TileGeom<2, 2>;
+  return ((nx + G::TX - 1) / G::TX) * ((ny + G::TY - 1) / G::TY);
+}
+
+hipError_t run_fill_noalloc(float* state, const float* src, const uint8_t* bc, int64_t nx,
+                            int64_t ny, hipStream_t stream) {
+  if (nx <= 0 || ny <= 0) return hipSuccess;
+  if (noalloc_fill_tiles(nx, ny) > kNoAllocMaxTiles) return hipErrorInvalidValue;
+  void* ws = nullptr;
+  hipError_t e = scratch_address(&ws);
+  if (e != hipSuccess) return e;
+  FillArgs a{};
+  a.v.state = state;
+  a.v.src = src;
+  a.v.bc = bc;
+  a.v.nx = nx;
+  a.v.ny = ny;
+  return launch_fill<TileGeom<2, 2>, kModeFill>(a, stream, nullptr, ws);
 }
 
 }  // namespace tt

@@ -55,6 +55,16 @@ hipError_t run_fill_gated(float* state, const float* src, const uint8_t* bc, int
                           int64_t ny, int mode, TileShape ts, hipStream_t stream,
                           RunStats* stats, const unsigned* gate, unsigned gate_bits);
 
+// fillsinks (work-queue algorithm, 128 x 128 tiles) with its queue
+// workspace in a fixed per-GPU device array instead of an allocation:
+// allocates nothing. Grids of more than kNoAllocMaxTiles tiles are
+// rejected (hipErrorInvalidValue); calls overlapping on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;  // 4 MiB per GPU
+constexpr int64_t kNoAllocMaxTiles = (int64_t)((kNoAllocScratchBytes - 64 * 4) / 16);
+int64_t noalloc_fill_tiles(int64_t nx, int64_t ny);
+hipError_t run_fill_noalloc(float* state, const float* src, const uint8_t* bc, int64_t nx,
+                            int64_t ny, hipStream_t stream);
+
 // Fast-sweeping excess topography: u (output) from dem and slopes.
 hipError_t run_fsm(float* u, const float* dem, const float* slopes, float cellsize, int64_t nx,
                    int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
