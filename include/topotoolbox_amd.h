@@ -36,7 +36,23 @@ extern "C" {
 #endif
 
 #define TOPOTOOLBOX_AMD_VERSION_MAJOR 0
-#define TOPOTOOLBOX_AMD_VERSION_MINOR 1
+#define TOPOTOOLBOX_AMD_VERSION_MINOR 2
+
+/* The reference's visibility and version macros (reference
+   include/topotoolbox.h:10-32), so that a caller switching headers keeps
+   them. The bodies equal the reference's on Linux (an empty
+   TOPOTOOLBOX_API: the symbols have default ELF visibility), so both
+   headers can be included in one translation unit, in either order. The
+   version is the reference interface these entry points implement. */
+#ifndef TOPOTOOLBOX_API
+#define TOPOTOOLBOX_API
+#endif
+#define TOPOTOOLBOX_VERSION_MAJOR 3
+#define TOPOTOOLBOX_VERSION_MINOR 0
+#define TOPOTOOLBOX_VERSION_PATCH 0
+#ifndef TOPOTOOLBOX_OPENMP_VERSION
+#define TOPOTOOLBOX_OPENMP_VERSION 0
+#endif
 
 /* ---------------------------------------------------------------------
    Part 1: drop-in replacements (host pointers)
@@ -44,20 +60,24 @@ extern "C" {
 
 /* Replaces has_topotoolbox, reference include/topotoolbox.h:47-48
    (src/topotoolbox.c:4-5). Always returns 1. */
+TOPOTOOLBOX_API
 int has_topotoolbox(void);
 
 /* Replaces fillsinks, reference include/topotoolbox.h:292-293
    (src/fillsinks.c:22-48). `dem` is read only (the reference mutates and
    restores it; the observable contract is the same). */
+TOPOTOOLBOX_API
 void fillsinks(float *output, float *dem, uint8_t *bc, ptrdiff_t dims[2]);
 
 /* Replaces reconstruct, reference include/topotoolbox.h:1055-1056
    (src/reconstruct.c:270-279). Runs to convergence; the reference stops
    after 1000 iterations, which matters only for inputs that need more. */
+TOPOTOOLBOX_API
 void reconstruct(float *marker, float *mask, ptrdiff_t dims[2]);
 
 /* Replaces excesstopography_fsm2d, reference
    include/topotoolbox.h:607-609 (src/excesstopography.c:47-111). */
+TOPOTOOLBOX_API
 void excesstopography_fsm2d(float *excess, float *dem, float *threshold_slopes,
                             float cellsize, ptrdiff_t dims[2]);
 
@@ -65,6 +85,7 @@ void excesstopography_fsm2d(float *excess, float *dem, float *threshold_slopes,
    (src/fillsinks.c:50-77). Same output as fillsinks (the reference's hybrid
    algorithm reaches the same reconstruction); `queue` is scratch the
    reference needs for its FIFO and is not used here. */
+TOPOTOOLBOX_API
 void fillsinks_hybrid(float *output, ptrdiff_t *queue, float *dem, uint8_t *bc,
                       ptrdiff_t dims[2]);
 
@@ -72,6 +93,7 @@ void fillsinks_hybrid(float *output, ptrdiff_t *queue, float *dem, uint8_t *bc,
    (src/reconstruct.c:291-312). Same output as reconstruct; `queue` is not
    used. Returns 0 (converged), or 1 if the call failed (the reference
    returns 1 when its queue overflowed twice without converging). */
+TOPOTOOLBOX_API
 int reconstruct_hybrid(float *marker, ptrdiff_t *queue, float *mask, ptrdiff_t dims[2]);
 
 /* Replaces excesstopography_fmm2d, reference include/topotoolbox.h:694-697
@@ -80,28 +102,34 @@ int reconstruct_hybrid(float *marker, ptrdiff_t *queue, float *mask, ptrdiff_t d
    reference's own tests require them to agree within 1e-4
    (test/excesstopography.cpp:69-82,133); `heap` and `back` (priority-queue
    scratch of the reference) are not used. */
+TOPOTOOLBOX_API
 void excesstopography_fmm2d(float *excess, ptrdiff_t *heap, ptrdiff_t *back, float *dem,
                             float *threshold_slopes, float cellsize, ptrdiff_t dims[2]);
 
 /* Replaces identifyflats, reference include/topotoolbox.h:405
    (src/identifyflats.c:32-121). output[p] & 1: flat, & 2: sill, & 4:
    presill; returns the number of flats. */
+TOPOTOOLBOX_API
 ptrdiff_t identifyflats(int32_t *output, float *dem, ptrdiff_t dims[2]);
 
 /* Replace min_filter and max_filter, reference include/topotoolbox.h:101-102
    and :155-156 (src/value_filters.c:9-59, 61-110). The structuring element
    is se_dims[0] x se_dims[1] x se_dims[2] bytes (slices folded in turn);
    the result has the reference's bits, including the sign of zeros. */
+TOPOTOOLBOX_API
 void min_filter(float *output, float *dem, uint8_t *structuring_element, ptrdiff_t io_dims[2],
                 ptrdiff_t se_dims[3]);
+TOPOTOOLBOX_API
 void max_filter(float *output, float *dem, uint8_t *structuring_element, ptrdiff_t io_dims[2],
                 ptrdiff_t se_dims[3]);
 
 /* Replace min_filter_square and max_filter_square, reference
    include/topotoolbox.h:204-205 and :253-254 (src/value_filters.c:112-178,
    180-246). `tmp` receives the pass along dims[0], as in the reference. */
+TOPOTOOLBOX_API
 void min_filter_square(float *output, float *dem, float *tmp, uint8_t width,
                        ptrdiff_t io_dims[2]);
+TOPOTOOLBOX_API
 void max_filter_square(float *output, float *dem, float *tmp, uint8_t width,
                        ptrdiff_t io_dims[2]);
 
@@ -110,8 +138,11 @@ void max_filter_square(float *output, float *dem, float *tmp, uint8_t width,
    flat component - its own depth) + 0.1, depth = filled - original;
    conncomps = the component's label, the same pixel the reference's scan
    picks (a deepest pixel of the component). Non-flats get 0 and 0. Flats
-   are expected as identifyflats makes them (none on the grid border), with
-   finite depths; nx * ny must be below 2^32. */
+   are expected as identifyflats makes them (none on the grid border).
+   NaN depths (flats of INF - INF) are handled as the reference handles
+   them: a NaN depth is never "strictly deeper", so the label follows the
+   reference's scan and the cost is NaN. nx * ny must be below 2^32. */
+TOPOTOOLBOX_API
 void gwdt_computecosts(float *costs, ptrdiff_t *conncomps, int32_t *flats, float *original_dem,
                        float *filled_dem, ptrdiff_t dims[2]);
 
@@ -193,8 +224,9 @@ int tt_identifyflats_dev(int32_t *output, const float *dem, ptrdiff_t nx, ptrdif
                          void *stream, int64_t *count);
 
 /* gwdt_computecosts on device buffers. Waits for completion (the number
-   of tie-breaking steps to replay is read back once). Returns nonzero with
-   a message for a NaN depth at a flat or nx * ny >= 2^32. */
+   of tie-breaking steps to replay is read back once). NaN depths are
+   handled as in the reference (see gwdt_computecosts). Returns nonzero
+   with a message for nx * ny >= 2^32. */
 int tt_gwdt_computecosts_dev(float *costs, int64_t *conncomps, const int32_t *flats,
                              const float *original_dem, const float *filled_dem, ptrdiff_t nx,
                              ptrdiff_t ny, void *stream);
@@ -290,6 +322,33 @@ int tt_set_fill_method(int method);
    TT_FILL_QUEUE on the device, with the same result. For tests of that
    fallback; returns nonzero outside the range. */
 int tt_set_fill_edge_cap(int cap);
+
+/* Device errors of asynchronous calls. A call made without `stats` returns
+   before its kernels end, so a device error (a bounded queue wait that
+   expired: bit 1) cannot be in its return code. Every persistent launch
+   also ORs its error bits into a word per GPU, which this call reads for
+   the current GPU after synchronising the device (clearing it when
+   `clear` is nonzero). No counterpart in the reference, whose functions
+   are synchronous. Returns 0 and stores the bits in *bits. The host-pointer
+   drop-ins check (and clear) the word themselves before returning. */
+int tt_device_error(int clear, uint32_t *bits);
+
+/* Upper bound, in ticks of the 100 MHz wall clock, on one bounded device
+   wait of later launches (process-wide; 0 restores the default of 10 s).
+   No counterpart in the reference: for tests of the error path, which set
+   a tiny bound so that a wait gives up and the launch ends early with its
+   error bit set. Always returns 0. */
+int tt_set_wait_limit(uint64_t ticks);
+
+/* Frees the device memory the host-pointer drop-ins keep between calls
+   (one grow-only buffer per GPU, sized by the largest call so far). No
+   counterpart in the reference. */
+void tt_host_workspace_release(void);
+
+/* Hash of the sources the library was built from (the Python loader
+   compares it with the sources beside it and refuses a stale binary). No
+   counterpart in the reference. */
+const char *tt_build_hash(void);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *tt_last_error(void);

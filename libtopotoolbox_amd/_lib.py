@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .build import LIB_PATH
+from .build import LIB_PATH, source_hash
 
 
 class LibraryNotBuilt(RuntimeError):
@@ -118,6 +118,10 @@ SIGNATURES = {
     "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "tt_set_fill_method": (ctypes.c_int, [ctypes.c_int]),
     "tt_set_fill_edge_cap": (ctypes.c_int, [ctypes.c_int]),
+    "tt_device_error": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]),
+    "tt_set_wait_limit": (ctypes.c_int, [ctypes.c_uint64]),
+    "tt_host_workspace_release": (None, []),
+    "tt_build_hash": (ctypes.c_char_p, []),
     "tt_last_error": (ctypes.c_char_p, []),
 }
 
@@ -135,6 +139,15 @@ def lib() -> ctypes.CDLL:
             L = ctypes.CDLL(LIB_PATH)
         except OSError as e:
             raise LibraryNotBuilt(f"cannot load {LIB_PATH}: {e}") from e
+        stamp = getattr(L, "tt_build_hash", None)
+        if stamp is None:
+            raise LibraryNotBuilt(f"{LIB_PATH} carries no source stamp: rebuild it")
+        stamp.restype = ctypes.c_char_p
+        built, now = stamp().decode(), source_hash()
+        if built != now:
+            raise LibraryNotBuilt(
+                f"{LIB_PATH} was built from other sources (stamp {built}, sources {now}): "
+                "rebuild it with `python -m libtopotoolbox_amd.build`")
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)
             if fn is None:
@@ -159,6 +172,23 @@ def last_error() -> str:
 def check(status: int, what: str) -> None:
     if status != 0:
         raise TTError(f"{what} failed (status {status}): {last_error()}")
+
+
+def device_error(clear: bool = True) -> int:
+    """Error bits that persistent launches on the current GPU recorded since
+    the last clear (synchronises the device). 1: a bounded queue wait
+    expired, so some result is incomplete."""
+    bits = ctypes.c_uint32(0)
+    check(lib().tt_device_error(1 if clear else 0, ctypes.byref(bits)), "tt_device_error")
+    return int(bits.value)
+
+
+def check_device_error(what: str) -> None:
+    """Raises TTError if a launch on the current GPU recorded an error."""
+    bits = device_error(clear=True)
+    if bits:
+        raise TTError(f"{what}: device error bits 0x{bits:x} (1: a queue wait exceeded its "
+                      "time limit; results are incomplete)")
 
 
 def dims(nx: int, ny: int):

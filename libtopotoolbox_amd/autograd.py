@@ -10,12 +10,17 @@ over the C ABI's adjoint entry points).
 * ``ExcessTopography``: the backward solves the adjoint of the converged
   Godunov system by reverse relaxation sweeps, giving gradients with
   respect to the DEM and to the threshold slopes.
+
+Each backward ends by reading the GPU's device-error word (a device
+synchronisation): a forward or backward launch whose bounded queue wait
+expired raises ``TTError`` there instead of returning incomplete
+gradients.
 """
 from __future__ import annotations
 
 import torch
 
-from . import ops
+from . import _lib, ops
 
 
 class FillSinks(torch.autograd.Function):
@@ -30,6 +35,8 @@ class FillSinks(torch.autograd.Function):
     def backward(ctx, g_out: torch.Tensor):
         (spill,) = ctx.saved_tensors
         g = ops.fillsinks_backward(spill, g_out.contiguous().to(torch.float32))
+        with torch.cuda.device(g.device):
+            _lib.check_device_error("fillsinks forward/backward")
         return g, None
 
 
@@ -49,6 +56,8 @@ class ExcessTopography(torch.autograd.Function):
         u, dem, s = ctx.saved_tensors
         g_dem, g_s = ops.excesstopography_fsm2d_backward(
             u, dem, s, ctx.cellsize, g_out.contiguous().to(torch.float32))
+        with torch.cuda.device(g_dem.device):
+            _lib.check_device_error("excesstopography_fsm2d forward/backward")
         return g_dem, g_s, None
 
 

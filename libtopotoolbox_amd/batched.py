@@ -165,7 +165,10 @@ class Batch:
 
     # ------------------------------------------------------------ results
     def outputs(self) -> dict:
-        """Per-item 2-D views of the results."""
+        """Per-item 2-D views of the results (after checking, with a device
+        synchronisation, that no launch recorded a device error)."""
+        with torch.cuda.device(self.output.device):
+            _lib.check_device_error("batch")
         r = {"output": self.views(self.output), "g_dem": self.views(self.g_dem)}
         if self.op == "fillsinks":
             r["spill"] = self.views(self.spill)

@@ -7,6 +7,8 @@ Run as ``python -m libtopotoolbox_amd.build`` or through
 """
 from __future__ import annotations
 
+import glob
+import hashlib
 import os
 import shutil
 import subprocess
@@ -41,6 +43,20 @@ FLAGS = [
 ]
 
 
+def source_hash() -> str:
+    """SHA-256 (first 32 hex digits) over the library's sources: every
+    csrc/*.hip and csrc/*.h and the public header. Stamped into the built
+    library (tt_build_hash) so a loader can refuse a stale binary."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h")) +
+                   glob.glob(os.path.join(ROOT, "include", "*.h")))
+    for path in files:
+        h.update(os.path.relpath(path, ROOT).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:32]
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -71,7 +87,7 @@ def build(verbose: bool = False) -> str:
     obj_dir = os.path.join(OUT_DIR, "obj")
     os.makedirs(obj_dir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
-    compile_flags = [f for f in FLAGS if f != "-shared"]
+    compile_flags = [f for f in FLAGS if f != "-shared"] + [f'-DTT_SOURCE_HASH="{source_hash()}"']
     jobs = []
     for src in sources():
         obj = os.path.join(obj_dir, os.path.basename(src) + ".o")

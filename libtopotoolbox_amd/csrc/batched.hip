@@ -440,6 +440,11 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
   a.q.state = reinterpret_cast<unsigned*>(a.q.ring + h.ntiles);
   a.q.visited = a.q.state + h.ntiles;
   a.q.ntiles = h.ntiles;
+  a.q.wait_limit = wait_limit_ticks();
+  {
+    const hipError_t se = sticky_error_word(&a.q.sticky);
+    if (se != hipSuccess) return se;
+  }
   a.q.stats = stats ? 1 : 0;
   const size_t cells = (size_t)h.cells;
   if (h.op == kBatchFill) {

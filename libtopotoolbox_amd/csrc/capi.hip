@@ -3,8 +3,11 @@
 #include <limits.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "../../include/topotoolbox_amd.h"
 #include "kernels.h"
+#include "workspace.h"
 
 namespace tt {
 const char* last_error();
@@ -46,6 +49,68 @@ int device_error(const char* where, uint32_t bits) {
   return 1000 + (int)bits;
 }
 
+// Device memory of the host-pointer drop-ins: one grow-only buffer per GPU,
+// reused by every call (no allocation per call once it is large enough)
+// and held until tt_host_workspace_release(). A call holds its GPU's lock
+// from the first copy in to the last copy out, so concurrent drop-in calls
+// on one GPU run one after the other.
+constexpr int kMaxDevices = 64;
+struct HostCache {
+  std::mutex mu;
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+HostCache g_host_cache[kMaxDevices];
+
+// Device buffers of one drop-in call, carved from its GPU's cache.
+class HostStage {
+ public:
+  // sizes: bytes of each buffer (each starts 256-byte aligned)
+  hipError_t acquire(const size_t* sizes, int count) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    cache_ = &g_host_cache[dev];
+    lock_ = std::unique_lock<std::mutex>(cache_->mu);
+    size_t total = 0;
+    for (int k = 0; k < count; k++) total += (sizes[k] + 255) & ~(size_t)255;
+    if (total > cache_->bytes) {
+      if (cache_->p) {
+        e = hipFree(cache_->p);
+        cache_->p = nullptr;
+        cache_->bytes = 0;
+        if (e != hipSuccess) return e;
+      }
+      e = hipMalloc(&cache_->p, total ? total : 256);
+      if (e != hipSuccess) return e;
+      cache_->bytes = total;
+    }
+    char* b = static_cast<char*>(cache_->p);
+    for (int k = 0; k < count; k++) {
+      ptr_[k] = b;
+      b += (sizes[k] + 255) & ~(size_t)255;
+    }
+    return hipSuccess;
+  }
+  void* operator[](int k) const { return ptr_[k]; }
+
+ private:
+  HostCache* cache_ = nullptr;
+  std::unique_lock<std::mutex> lock_;
+  void* ptr_[8] = {};
+};
+
+// Device errors of a synchronous drop-in call (its kernels have finished
+// when this runs).
+int sticky_check(const char* where) {
+  unsigned bits = 0;
+  hipError_t e = tt::take_sticky_error(true, &bits);
+  if (e != hipSuccess) return fail(where, e);
+  if (bits) return device_error(where, bits);
+  return 0;
+}
+
 // Drop-in failures are reported once on stderr: the reference API has no
 // error channel, and a silent failure would hand back garbage.
 void report(const char* fn) {
@@ -56,14 +121,6 @@ void report(const char* fn) {
   }
 }
 
-// Host staging for the drop-in entry points.
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-};
 
 bool dims_ok(const ptrdiff_t* dims, const char* fn) {
   if (!dims || dims[0] < 0 || dims[1] < 0) {
@@ -78,6 +135,11 @@ bool dims_ok(const ptrdiff_t* dims, const char* fn) {
 extern "C" {
 
 int has_topotoolbox(void) { return 1; }
+
+#ifndef TT_SOURCE_HASH
+#define TT_SOURCE_HASH "unstamped"
+#endif
+const char* tt_build_hash(void) { return TT_SOURCE_HASH; }
 
 const char* tt_last_error(void) { return tt::last_error(); }
 
@@ -257,14 +319,9 @@ int tt_gwdt_computecosts_dev(float* costs, int64_t* conncomps, const int32_t* fl
     tt::set_error("tt_gwdt_computecosts_dev: dims must be >= 0 with nx * ny < 2^32 - 1");
     return 1;
   }
-  unsigned bits = 0;
   hipError_t e = tt::run_gwdt_computecosts(costs, conncomps, flats, original_dem, filled_dem, nx,
-                                           ny, (hipStream_t)stream, &bits);
+                                           ny, (hipStream_t)stream);
   if (e != hipSuccess) return fail("tt_gwdt_computecosts_dev", e);
-  if (bits) {
-    tt::set_error("tt_gwdt_computecosts_dev: a flat has a NaN depth (filled - original)");
-    return 2;
-  }
   return 0;
 }
 
@@ -324,79 +381,71 @@ int tt_batch_run_dev(const void* plan_host, const void* plan_dev,
   return 0;
 }
 
+// Host-pointer drop-ins: copy in, run the device entry point on the null
+// stream, copy out (hipMemcpy waits for the kernels), then report device
+// errors. Device memory comes from the per-GPU cache (HostStage).
+#define TT_HOST_TRY(fn, expr)           \
+  do {                                  \
+    hipError_t e_ = (expr);             \
+    if (e_ != hipSuccess) {             \
+      fail(fn, e_);                     \
+      return report(fn);                \
+    }                                   \
+  } while (0)
+
 void fillsinks(float* output, float* dem, uint8_t* bc, ptrdiff_t dims[2]) {
-  if (!dims_ok(dims, "fillsinks")) return report("fillsinks");
+  const char* fn = "fillsinks";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn);
   const size_t n = (size_t)dims[0] * (size_t)dims[1];
   if (n == 0) return;
-  DevBuf d_out, d_dem, d_bc;
-  hipError_t e = d_out.alloc(n * 4);
-  if (e == hipSuccess) e = d_dem.alloc(n * 4);
-  if (e == hipSuccess) e = d_bc.alloc(n);
-  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_bc.p, bc, n, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail("fillsinks", e);
-    return report("fillsinks");
-  }
-  tt_stats st;
-  if (tt_fillsinks_dev((float*)d_out.p, (const float*)d_dem.p, (const uint8_t*)d_bc.p, dims[0],
-                       dims[1], nullptr, &st) != 0)
-    return report("fillsinks");
-  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail("fillsinks", e);
-    report("fillsinks");
-  }
+  HostStage st;
+  const size_t sz[3] = {n * 4, n * 4, n};
+  TT_HOST_TRY(fn, st.acquire(sz, 3));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[2], bc, n, hipMemcpyHostToDevice));
+  if (tt_fillsinks_dev((float*)st[0], (const float*)st[1], (const uint8_t*)st[2], dims[0],
+                       dims[1], nullptr, nullptr) != 0)
+    return report(fn);
+  TT_HOST_TRY(fn, hipMemcpy(output, st[0], n * 4, hipMemcpyDeviceToHost));
+  if (sticky_check(fn) != 0) report(fn);
 }
 
 void reconstruct(float* marker, float* mask, ptrdiff_t dims[2]) {
-  if (!dims_ok(dims, "reconstruct")) return report("reconstruct");
+  const char* fn = "reconstruct";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn);
   const size_t n = (size_t)dims[0] * (size_t)dims[1];
   if (n == 0) return;
-  DevBuf d_marker, d_mask;
-  hipError_t e = d_marker.alloc(n * 4);
-  if (e == hipSuccess) e = d_mask.alloc(n * 4);
-  if (e == hipSuccess) e = hipMemcpy(d_marker.p, marker, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_mask.p, mask, n * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail("reconstruct", e);
-    return report("reconstruct");
-  }
-  tt_stats st;
-  if (tt_reconstruct_dev((float*)d_marker.p, (const float*)d_mask.p, dims[0], dims[1], nullptr,
-                         &st) != 0)
-    return report("reconstruct");
-  e = hipMemcpy(marker, d_marker.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail("reconstruct", e);
-    report("reconstruct");
-  }
+  HostStage st;
+  const size_t sz[2] = {n * 4, n * 4};
+  TT_HOST_TRY(fn, st.acquire(sz, 2));
+  TT_HOST_TRY(fn, hipMemcpy(st[0], marker, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], mask, n * 4, hipMemcpyHostToDevice));
+  if (tt_reconstruct_dev((float*)st[0], (const float*)st[1], dims[0], dims[1], nullptr,
+                         nullptr) != 0)
+    return report(fn);
+  TT_HOST_TRY(fn, hipMemcpy(marker, st[0], n * 4, hipMemcpyDeviceToHost));
+  if (sticky_check(fn) != 0) report(fn);
 }
 
 void excesstopography_fsm2d(float* excess, float* dem, float* threshold_slopes, float cellsize,
                             ptrdiff_t dims[2]) {
-  if (!dims_ok(dims, "excesstopography_fsm2d")) return report("excesstopography_fsm2d");
+  const char* fn = "excesstopography_fsm2d";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn);
   const size_t n = (size_t)dims[0] * (size_t)dims[1];
   if (n == 0) return;
-  DevBuf d_u, d_dem, d_s;
-  hipError_t e = d_u.alloc(n * 4);
-  if (e == hipSuccess) e = d_dem.alloc(n * 4);
-  if (e == hipSuccess) e = d_s.alloc(n * 4);
-  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_s.p, threshold_slopes, n * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail("excesstopography_fsm2d", e);
-    return report("excesstopography_fsm2d");
-  }
-  tt_stats st;
-  if (tt_excesstopography_fsm2d_dev((float*)d_u.p, (const float*)d_dem.p, (const float*)d_s.p,
-                                    cellsize, dims[0], dims[1], nullptr, &st) != 0)
-    return report("excesstopography_fsm2d");
-  e = hipMemcpy(excess, d_u.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail("excesstopography_fsm2d", e);
-    report("excesstopography_fsm2d");
-  }
+  HostStage st;
+  const size_t sz[3] = {n * 4, n * 4, n * 4};
+  TT_HOST_TRY(fn, st.acquire(sz, 3));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[2], threshold_slopes, n * 4, hipMemcpyHostToDevice));
+  if (tt_excesstopography_fsm2d_dev((float*)st[0], (const float*)st[1], (const float*)st[2],
+                                    cellsize, dims[0], dims[1], nullptr, nullptr) != 0)
+    return report(fn);
+  TT_HOST_TRY(fn, hipMemcpy(excess, st[0], n * 4, hipMemcpyDeviceToHost));
+  if (sticky_check(fn) != 0) report(fn);
 }
 
 void fillsinks_hybrid(float* output, ptrdiff_t* queue, float* dem, uint8_t* bc,
@@ -407,7 +456,6 @@ void fillsinks_hybrid(float* output, ptrdiff_t* queue, float* dem, uint8_t* bc,
 
 int reconstruct_hybrid(float* marker, ptrdiff_t* queue, float* mask, ptrdiff_t dims[2]) {
   (void)queue;
-  tt::set_error("%s", "");
   reconstruct(marker, mask, dims);
   return tt::last_error()[0] ? 1 : 0;
 }
@@ -420,57 +468,74 @@ void excesstopography_fmm2d(float* excess, ptrdiff_t* heap, ptrdiff_t* back, flo
 }
 
 ptrdiff_t identifyflats(int32_t* output, float* dem, ptrdiff_t dims[2]) {
-  if (!dims_ok(dims, "identifyflats")) return report("identifyflats"), 0;
+  const char* fn = "identifyflats";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn), 0;
   const size_t n = (size_t)dims[0] * (size_t)dims[1];
   if (n == 0) return 0;
-  DevBuf d_out, d_dem;
-  hipError_t e = d_out.alloc(n * 4);
-  if (e == hipSuccess) e = d_dem.alloc(n * 4);
-  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail("identifyflats", e);
-    return report("identifyflats"), 0;
-  }
+  HostStage st;
+  const size_t sz[2] = {n * 4, n * 4};
+  hipError_t e = st.acquire(sz, 2);
+  if (e == hipSuccess) e = hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(fn, e), report(fn), 0;
   int64_t count = 0;
-  if (tt_identifyflats_dev((int32_t*)d_out.p, (const float*)d_dem.p, dims[0], dims[1], nullptr,
+  if (tt_identifyflats_dev((int32_t*)st[0], (const float*)st[1], dims[0], dims[1], nullptr,
                            &count) != 0)
-    return report("identifyflats"), 0;
-  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail("identifyflats", e);
-    report("identifyflats");
-  }
+    return report(fn), 0;
+  e = hipMemcpy(output, st[0], n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(fn, e), report(fn), 0;
   return (ptrdiff_t)count;
 }
 
 void gwdt_computecosts(float* costs, ptrdiff_t* conncomps, int32_t* flats, float* original_dem,
                        float* filled_dem, ptrdiff_t dims[2]) {
-  if (!dims_ok(dims, "gwdt_computecosts")) return report("gwdt_computecosts");
+  const char* fn = "gwdt_computecosts";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn);
   const size_t n = (size_t)dims[0] * (size_t)dims[1];
   if (n == 0) return;
-  DevBuf d_costs, d_cc, d_flats, d_orig, d_filled;
-  hipError_t e = d_costs.alloc(n * 4);
-  if (e == hipSuccess) e = d_cc.alloc(n * 8);
-  if (e == hipSuccess) e = d_flats.alloc(n * 4);
-  if (e == hipSuccess) e = d_orig.alloc(n * 4);
-  if (e == hipSuccess) e = d_filled.alloc(n * 4);
-  if (e == hipSuccess) e = hipMemcpy(d_flats.p, flats, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_orig.p, original_dem, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_filled.p, filled_dem, n * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail("gwdt_computecosts", e);
-    return report("gwdt_computecosts");
+  HostStage st;
+  const size_t sz[5] = {n * 4, n * 8, n * 4, n * 4, n * 4};
+  TT_HOST_TRY(fn, st.acquire(sz, 5));
+  TT_HOST_TRY(fn, hipMemcpy(st[2], flats, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[3], original_dem, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[4], filled_dem, n * 4, hipMemcpyHostToDevice));
+  if (tt_gwdt_computecosts_dev((float*)st[0], (int64_t*)st[1], (const int32_t*)st[2],
+                               (const float*)st[3], (const float*)st[4], dims[0], dims[1],
+                               nullptr) != 0)
+    return report(fn);
+  TT_HOST_TRY(fn, hipMemcpy(costs, st[0], n * 4, hipMemcpyDeviceToHost));
+  TT_HOST_TRY(fn, hipMemcpy(conncomps, st[1], n * 8, hipMemcpyDeviceToHost));
+}
+
+void tt_host_workspace_release(void) {
+  for (int d = 0; d < kMaxDevices; d++) {
+    HostCache& c = g_host_cache[d];
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (!c.p) continue;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    if (hipSetDevice(d) == hipSuccess) {
+      (void)hipFree(c.p);
+      c.p = nullptr;
+      c.bytes = 0;
+    }
+    (void)hipSetDevice(cur);
   }
-  if (tt_gwdt_computecosts_dev((float*)d_costs.p, (int64_t*)d_cc.p, (const int32_t*)d_flats.p,
-                               (const float*)d_orig.p, (const float*)d_filled.p, dims[0],
-                               dims[1], nullptr) != 0)
-    return report("gwdt_computecosts");
-  e = hipMemcpy(costs, d_costs.p, n * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(conncomps, d_cc.p, n * 8, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail("gwdt_computecosts", e);
-    report("gwdt_computecosts");
-  }
+}
+
+int tt_device_error(int clear, uint32_t* bits) {
+  tt::set_error("%s", "");
+  unsigned b = 0;
+  hipError_t e = tt::take_sticky_error(clear != 0, &b);
+  if (e != hipSuccess) return fail("tt_device_error", e);
+  if (bits) *bits = b;
+  return 0;
+}
+
+int tt_set_wait_limit(uint64_t ticks) {
+  tt::set_wait_limit_ticks(ticks);
+  return 0;
 }
 
 }  // extern "C"
@@ -479,6 +544,7 @@ namespace {
 
 void value_filter_host(const char* fn, int op, float* output, const float* dem,
                        const uint8_t* se, const ptrdiff_t* io_dims, const ptrdiff_t* se_dims) {
+  tt::set_error("%s", "");
   if (!dims_ok(io_dims, fn)) return report(fn);
   if (!se_dims || se_dims[0] < 0 || se_dims[1] < 0 || se_dims[2] < 0) {
     tt::set_error("%s: invalid se_dims", fn);
@@ -487,49 +553,32 @@ void value_filter_host(const char* fn, int op, float* output, const float* dem,
   const size_t n = (size_t)io_dims[0] * (size_t)io_dims[1];
   const size_t ns = (size_t)se_dims[0] * (size_t)se_dims[1] * (size_t)se_dims[2];
   if (n == 0) return;
-  DevBuf d_out, d_dem, d_se;
-  hipError_t e = d_out.alloc(n * 4);
-  if (e == hipSuccess) e = d_dem.alloc(n * 4);
-  if (e == hipSuccess) e = d_se.alloc(ns);
-  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess && ns) e = hipMemcpy(d_se.p, se, ns, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail(fn, e);
-    return report(fn);
-  }
-  if (tt_value_filter_dev(op, (float*)d_out.p, (const float*)d_dem.p, (const uint8_t*)d_se.p,
+  HostStage st;
+  const size_t sz[3] = {n * 4, n * 4, ns};
+  TT_HOST_TRY(fn, st.acquire(sz, 3));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice));
+  if (ns) TT_HOST_TRY(fn, hipMemcpy(st[2], se, ns, hipMemcpyHostToDevice));
+  if (tt_value_filter_dev(op, (float*)st[0], (const float*)st[1], (const uint8_t*)st[2],
                           se_dims, io_dims[0], io_dims[1], nullptr) != 0)
     return report(fn);
-  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail(fn, e);
-    report(fn);
-  }
+  TT_HOST_TRY(fn, hipMemcpy(output, st[0], n * 4, hipMemcpyDeviceToHost));
 }
 
 void value_filter_square_host(const char* fn, int op, float* output, const float* dem,
                               float* tmp, int width, const ptrdiff_t* io_dims) {
+  tt::set_error("%s", "");
   if (!dims_ok(io_dims, fn)) return report(fn);
   const size_t n = (size_t)io_dims[0] * (size_t)io_dims[1];
   if (n == 0) return;
-  DevBuf d_out, d_dem, d_tmp;
-  hipError_t e = d_out.alloc(n * 4);
-  if (e == hipSuccess) e = d_dem.alloc(n * 4);
-  if (e == hipSuccess) e = d_tmp.alloc(n * 4);
-  if (e == hipSuccess) e = hipMemcpy(d_dem.p, dem, n * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    fail(fn, e);
+  HostStage st;
+  const size_t sz[3] = {n * 4, n * 4, n * 4};
+  TT_HOST_TRY(fn, st.acquire(sz, 3));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice));
+  if (tt_value_filter_square_dev(op, (float*)st[0], (const float*)st[1], (float*)st[2], width,
+                                 io_dims[0], io_dims[1], nullptr) != 0)
     return report(fn);
-  }
-  if (tt_value_filter_square_dev(op, (float*)d_out.p, (const float*)d_dem.p, (float*)d_tmp.p,
-                                 width, io_dims[0], io_dims[1], nullptr) != 0)
-    return report(fn);
-  e = hipMemcpy(output, d_out.p, n * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && tmp) e = hipMemcpy(tmp, d_tmp.p, n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) {
-    fail(fn, e);
-    report(fn);
-  }
+  TT_HOST_TRY(fn, hipMemcpy(output, st[0], n * 4, hipMemcpyDeviceToHost));
+  if (tmp) TT_HOST_TRY(fn, hipMemcpy(tmp, st[2], n * 4, hipMemcpyDeviceToHost));
 }
 
 }  // namespace

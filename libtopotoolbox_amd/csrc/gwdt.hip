@@ -203,10 +203,14 @@ __global__ void gw_link(Grid g, Ws w) {
 // a root stored into A[p] itself could be overwritten by another thread's
 // path halving that read A[p]'s old parent just before, leaving a
 // non-root behind.
-// Only run starts are nodes of the union-find: every other flat points at
-// a run start (its own, or an ancestor after path halving in gw_link) and
-// is never on anyone's path, so the roots are found from the run starts
-// alone and the other flats take their run start's root.
+// Invariant after gw_link: every entry of A points at a run start. The
+// initial entries do (a flat points at the start of its run), and links and
+// path halving only ever store roots and grandparents, which are entries
+// of A and therefore run starts. A non-start flat can still be the `par`
+// of a unite() in gw_link, whose find_halving then rewrites its entry, but
+// only with a run start. So the roots are found from the run starts alone,
+// every other flat takes its run start's root, and F[A[p]] is defined for
+// every flat (F is filled for every run start).
 __global__ void gw_births(Grid g, Ws w) {
   GW_FOR_PIXELS {
     const long long p = j * g.nx + i;
@@ -528,8 +532,7 @@ dim3 pixel_grid(long long nx, long long ny) {
 
 hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t* flats,
                                  const float* original, const float* filled, int64_t nx,
-                                 int64_t ny, hipStream_t s, unsigned* error_bits) {
-  *error_bits = 0;
+                                 int64_t ny, hipStream_t s) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
   const size_t n = (size_t)nx * (size_t)ny;
   // per-pixel arrays in one workspace (the outputs are written only by the

@@ -135,11 +135,10 @@ hipError_t run_value_filter(int op, float* out, const float* dem, const uint8_t*
 hipError_t run_value_filter_square(int op, float* out, const float* dem, float* tmp, int width,
                                    int64_t nx, int64_t ny, hipStream_t stream);
 
-// gwdt_computecosts (gwdt.hip); synchronises the stream once. error_bits:
-// 1 = a flat whose depth (filled - original) is NaN.
+// gwdt_computecosts (gwdt.hip); synchronises the stream once.
 hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t* flats,
                                  const float* original, const float* filled, int64_t nx,
-                                 int64_t ny, hipStream_t stream, unsigned* error_bits);
+                                 int64_t ny, hipStream_t stream);
 
 // Thread-local error message set by the entry points.
 void set_error(const char* fmt, ...);

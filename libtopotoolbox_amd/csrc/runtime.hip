@@ -4,9 +4,59 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "workspace.h"
 
 namespace tt {
+
+// Error bits of every launch on a GPU since the host last cleared them
+// (device errors of calls that return before their kernels finish).
+static __device__ unsigned sticky_error;
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_sticky_mu;
+unsigned* g_sticky_dev[kMaxDevices];
+std::atomic<unsigned long long> g_wait_limit{kWaitTimeoutTicks};
+}  // namespace
+
+hipError_t sticky_error_word(unsigned** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_sticky_mu);
+  if (!g_sticky_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(sticky_error));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_sticky_dev[dev] = static_cast<unsigned*>(p);
+  }
+  *out = g_sticky_dev[dev];
+  return hipSuccess;
+}
+
+unsigned long long wait_limit_ticks() { return g_wait_limit.load(); }
+
+void set_wait_limit_ticks(unsigned long long ticks) {
+  g_wait_limit.store(ticks ? ticks : kWaitTimeoutTicks);
+}
+
+hipError_t take_sticky_error(bool clear, unsigned* bits) {
+  *bits = 0;
+  unsigned* p = nullptr;
+  hipError_t e = sticky_error_word(&p);
+  if (e != hipSuccess) return e;
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(bits, p, sizeof(unsigned), hipMemcpyDeviceToHost);
+  if (e != hipSuccess || !clear || *bits == 0) return e;
+  const unsigned zero = 0;
+  return hipMemcpy(p, &zero, sizeof(unsigned), hipMemcpyHostToDevice);
+}
 
 __global__ void zero_words(unsigned* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,6 +140,9 @@ hipError_t Workspace::attach(void* mem, int ntiles, int ntx, int nty, hipStream_
   q.state = reinterpret_cast<unsigned*>(q.ring + ntiles);
   q.visited = q.state + ntiles;
   q.ntiles = ntiles;
+  q.wait_limit = wait_limit_ticks();
+  hipError_t e = sticky_error_word(&q.sticky);
+  if (e != hipSuccess) return e;
   int blocks = (ntiles + 255) / 256;
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;

@@ -56,8 +56,8 @@ enum : unsigned int {
   kErrTimeout = 1u,
 };
 
-// Upper bound on one wait (wall-clock ticks of the 100 MHz constant
-// clock): 10 s.
+// Default upper bound on one wait (wall-clock ticks of the 100 MHz
+// constant clock): 10 s. A launch carries its own bound (TileQueue).
 constexpr unsigned long long kWaitTimeoutTicks = 1000000000ull;
 
 __device__ inline unsigned ld_acquire(const unsigned* p) {
@@ -79,7 +79,19 @@ struct TileQueue {
   int stats;
   // the Sync block visits report their statistics to, or null
   __device__ Sync* stats_sink() const { return stats ? sync : nullptr; }
+  // per-GPU word that keeps the error bits of every launch until the host
+  // reads and clears it (may be null)
+  unsigned* sticky;
+  // upper bound on one wait, in wall-clock ticks
+  unsigned long long wait_limit;
 };
+
+// Records error bits in the launch's control block (which stops the grid)
+// and in the per-GPU sticky word.
+__device__ inline void raise_error(const TileQueue& q, unsigned bits) {
+  atomicOr(&q.sync->error, bits);
+  if (q.sticky) atomicOr(q.sticky, bits);
+}
 
 enum : unsigned { kQueued = 1u, kRunning = 2u };
 
@@ -123,8 +135,8 @@ __device__ inline void fill_slot(const TileQueue& q, unsigned pos, int t) {
   const unsigned long long t0 = wall_clock64();
   while (__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) {
     __builtin_amdgcn_s_sleep(1);
-    if (wall_clock64() - t0 > kWaitTimeoutTicks) {
-      atomicOr(&q.sync->error, kErrTimeout);
+    if (wall_clock64() - t0 > q.wait_limit) {
+      raise_error(q, kErrTimeout);
       return;
     }
   }
@@ -158,8 +170,8 @@ __device__ inline int pop_tile(const TileQueue& q) {
     }
     if ((spin & 7u) == 7u) {
       if (ld_relaxed(&s->pending) == 0u || ld_relaxed(&s->error) != 0u) return -1;
-      if (wall_clock64() - t0 > kWaitTimeoutTicks) {
-        atomicOr(&s->error, kErrTimeout);
+      if (wall_clock64() - t0 > q.wait_limit) {
+        raise_error(q, kErrTimeout);
         return -1;
       }
     }

@@ -37,6 +37,16 @@ struct Workspace {
   hipError_t finish(RunStats* st);
 };
 
+// Device address of this GPU's sticky error word (looked up once per
+// device) and the wait bound new launches carry; both fill a TileQueue.
+hipError_t sticky_error_word(unsigned** p);
+unsigned long long wait_limit_ticks();
+// Sets the wait bound of later launches (0: the default, 10 s).
+void set_wait_limit_ticks(unsigned long long ticks);
+// Reads the current GPU's sticky error word after synchronising the
+// device, and clears it when `clear` is set.
+hipError_t take_sticky_error(bool clear, unsigned* bits);
+
 // Workgroups of the persistent grid: co-resident blocks, capped by tiles.
 hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* grid);
 

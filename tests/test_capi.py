@@ -57,3 +57,31 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         pass
     else:
         raise AssertionError("loading an absent library must raise")
+
+
+def test_stale_library_is_refused(monkeypatch):
+    """The loader compares the source stamp built into the library with the
+    sources beside it."""
+    from libtopotoolbox_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "source_hash", lambda: "0" * 32)
+    try:
+        _lib.lib()
+    except _lib.LibraryNotBuilt as e:
+        assert "built from other sources" in str(e)
+    else:
+        raise AssertionError("a library built from other sources must be refused")
+    monkeypatch.undo()
+    _lib._lib = None
+    assert _lib.lib().has_topotoolbox() == 1
+
+
+def test_noalloc_rejects_oversized_grid_without_gpu():
+    """The size check of the zero-allocation entry point runs before any
+    device call: 600 x 600 tiles of 128 x 128 exceed its workspace."""
+    from libtopotoolbox_amd import _lib
+
+    L = _lib.lib()
+    assert L.tt_fillsinks_noalloc_dev(None, None, None, 128 * 600, 128 * 600, None) == 2
+    assert "at most 262128" in _lib.last_error()
