@@ -166,6 +166,19 @@ def test_fill_methods(cuda, oracle_lib, method, cap):
             for tile in TILES:
                 out, _ = gpu_fill(dem, bc, tile)
                 assert_bitwise(out, want, f"{method} cap {cap} shape {shape} tile {tile}")
+        for name in ["fillsinks_random_s0", "fillsinks_nan_raw", "fillsinks_nan_convention",
+                     "fillsinks_flats"]:
+            g = load_golden(name)
+            out, _ = gpu_fill(g["dem"], g["bc"])
+            assert_bitwise(out, g["out"], f"{method} cap {cap} golden {name}")
+        # 1024^2 with 5% single-cell pits and interior sources: 64 tiles
+        dem = oracle_lib.grid_random(1024, (1024, 1030))
+        pits = rng.uniform(size=dem.shape) < 0.05
+        dem[pits] -= rng.uniform(0, 20, size=int(pits.sum())).astype(np.float32)
+        dem[rng.uniform(size=dem.shape) < 0.002] = np.nan
+        bc = edge_bc(dem.shape) | (rng.uniform(size=dem.shape) < 0.0005).astype(np.uint8)
+        out, _ = gpu_fill(dem, bc)
+        assert_bitwise(out, oracle_lib.fillsinks_pf(dem, bc), f"{method} cap {cap} 1024 pits")
         mask = oracle_lib.grid_random(78, (150, 190))
         mask[rng.uniform(size=mask.shape) < 0.03] = np.nan
         marker = np.where(rng.uniform(size=mask.shape) < 0.05, mask + 5.0,
@@ -178,6 +191,8 @@ def test_fill_methods(cuda, oracle_lib, method, cap):
             tt.set_tile_shape(*tile)
             out = tt.reconstruct(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda())
             assert_bitwise(out.cpu().numpy(), want, f"{method} cap {cap} reconstruct {tile}")
+        # an overflow fallback is not a device error
+        assert _lib.device_error(clear=True) == 0
     finally:
         ops.set_fill_method("queue")
         _lib.lib().tt_set_fill_edge_cap(2048)
