@@ -306,23 +306,6 @@ int tt_batch_run_dev(const void *plan_host, const void *plan_dev,
    topography 2x1 (the fastest measured at 8192^2). */
 int tt_set_tile_shape(int qx, int qy);
 
-/* Algorithm of fillsinks for subsequent calls on this thread (both give
-   the same bits): TT_FILL_QUEUE (default) revisits tiles through the work
-   queue until nothing changes; TT_FILL_GRAPH labels each tile's drainage
-   to its ring cells, solves the small graph of ring cells over all tiles,
-   then fills each tile once. reconstruct always uses the work
-   queue (its markers can seed cells above their own level, which the
-   tile graph does not model). Returns nonzero for an unknown method. */
-#define TT_FILL_GRAPH 0
-#define TT_FILL_QUEUE 1
-int tt_set_fill_method(int method);
-
-/* Label edges one tile may hold in TT_FILL_GRAPH (1..2048, default 2048;
-   process-wide). A tile that needs more makes the call fall back to
-   TT_FILL_QUEUE on the device, with the same result. For tests of that
-   fallback; returns nonzero outside the range. */
-int tt_set_fill_edge_cap(int cap);
-
 /* Device errors of asynchronous calls. A call made without `stats` returns
    before its kernels end, so a device error (a bounded queue wait that
    expired: bit 1) cannot be in its return code. Every persistent launch

@@ -30,7 +30,6 @@ from .ops import (
     reconstruct,
     reconstruct_host,
     reconstruct_hybrid_host,
-    set_fill_method,
     set_tile_shape,
 )
 
@@ -56,7 +55,6 @@ __all__ = [
     "reconstruct",
     "reconstruct_host",
     "reconstruct_hybrid_host",
-    "set_fill_method",
     "set_tile_shape",
 ]
 

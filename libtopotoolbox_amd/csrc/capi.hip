@@ -21,7 +21,6 @@ thread_local tt::TileShape g_shape;
 thread_local bool g_shape_set = false;
 
 tt::TileShape shape_for(tt::TileShape fallback) { return g_shape_set ? g_shape : fallback; }
-thread_local int g_fill_method = tt::kFillQueue;
 
 void to_public(const tt::RunStats& s, tt_stats* o) {
   o->requeues = s.requeues;
@@ -154,23 +153,6 @@ int tt_set_tile_shape(int qx, int qy) {
   return 0;
 }
 
-int tt_set_fill_method(int method) {
-  if (method != tt::kFillGraph && method != tt::kFillQueue) {
-    tt::set_error("tt_set_fill_method: unknown method %d", method);
-    return 1;
-  }
-  g_fill_method = method;
-  return 0;
-}
-
-int tt_set_fill_edge_cap(int cap) {
-  if (!tt::set_graph_edge_cap(cap)) {
-    tt::set_error("tt_set_fill_edge_cap: cap must be in 1..2048");
-    return 1;
-  }
-  return 0;
-}
-
 int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff_t nx,
                      ptrdiff_t ny, void* stream, tt_stats* stats) {
   tt::set_error("%s", "");
@@ -180,7 +162,6 @@ int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff
   }
   tt::RunStats st;
   hipError_t e = tt::run_fill(output, dem, bc, nx, ny, tt::kModeFill, shape_for({2, 2}),
-                              g_fill_method,
                               (hipStream_t)stream, stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_fillsinks_dev", e);
   if (stats) {
@@ -219,7 +200,7 @@ int tt_reconstruct_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t
   }
   tt::RunStats st;
   hipError_t e = tt::run_fill(marker, mask, nullptr, nx, ny, tt::kModeRecon, shape_for({2, 2}),
-                              g_fill_method, (hipStream_t)stream, stats ? &st : nullptr);
+                              (hipStream_t)stream, stats ? &st : nullptr);
   if (e != hipSuccess) return fail("tt_reconstruct_dev", e);
   if (stats) {
     to_public(st, stats);

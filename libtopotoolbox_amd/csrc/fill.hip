@@ -12,16 +12,11 @@ struct FillArgs {
   int ntx, nty;
   TileQueue q;
   unsigned max_local;
-  const unsigned* gate;  // if set: run only when *gate has one of gate_bits
-  unsigned gate_bits;
 };
 
 template <class G, int MODE>
 __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
   __shared__ FillLds<G, MODE> lds;
-  if (a.gate && !(__hip_atomic_load(a.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-                  a.gate_bits))
-    return;
   run_queue(a.q, GridTopo{a.ntx, a.nty}, [&](int t, bool first, unsigned nvis) {
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
     return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds);
@@ -55,9 +50,8 @@ static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, Ru
   }
 }
 
-hipError_t run_fill_gated(float* state, const float* src, const uint8_t* bc, int64_t nx,
-                          int64_t ny, int mode, TileShape ts, hipStream_t stream,
-                          RunStats* stats, const unsigned* gate, unsigned gate_bits) {
+hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
+                    int mode, TileShape ts, hipStream_t stream, RunStats* stats) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
   FillArgs a{};
   a.v.state = state;
@@ -65,17 +59,8 @@ hipError_t run_fill_gated(float* state, const float* src, const uint8_t* bc, int
   a.v.bc = bc;
   a.v.nx = nx;
   a.v.ny = ny;
-  a.gate = gate;
-  a.gate_bits = gate_bits;
   if (mode == kModeFill) return dispatch_fill<kModeFill>(a, ts, stream, stats);
   return dispatch_fill<kModeRecon>(a, ts, stream, stats);
-}
-
-hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
-                    int mode, TileShape ts, int method, hipStream_t stream, RunStats* stats) {
-  if (method == kFillGraph && mode == kModeFill)
-    return run_fill_graph(state, src, bc, nx, ny, ts, stream, stats);
-  return run_fill_gated(state, src, bc, nx, ny, mode, ts, stream, stats, nullptr, 0u);
 }
 
 hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc,
@@ -91,8 +76,7 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   a.v.ny = ny;
   // exact levels first (float relaxation, the fast path), then the
   // (level, index) keys from (level, no index): only spill indices move
-  hipError_t e = run_fill_gated(out, dem, bc, nx, ny, kModeFill, TileShape{2, 2}, stream,
-                                nullptr, nullptr, 0u);
+  hipError_t e = run_fill(out, dem, bc, nx, ny, kModeFill, TileShape{2, 2}, stream, nullptr);
   if (e != hipSuccess) return e;
   a.v.levels = true;
   // (level, index) keys live in one 64-bit array between visits; the

@@ -30,30 +30,10 @@ enum FillMode : int {
   kModeRecon = 1,  // reconstruct: state = marker (in place), src = mask
 };
 
-// Fill / reconstruct algorithms (same results bit for bit).
-enum FillMethod : int {
-  kFillGraph = 0,  // fillsinks by a tile graph: label flood, ring-graph solve, final fill
-  kFillQueue = 1,  // tiles revisited through the work queue until converged
-};
-
 // Sink filling / reconstruction. `stats` (host) may be null; when set the
 // call synchronises the stream to read the control block.
 hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
-                    int mode, TileShape ts, int method, hipStream_t stream, RunStats* stats);
-
-// The tile-graph algorithm (fill_graph.hip), fillsinks only.
-hipError_t run_fill_graph(float* state, const float* src, const uint8_t* bc, int64_t nx,
-                          int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
-
-// Label edges a tile may hold in the tile-graph algorithm (1..2048; a
-// tile needing more makes the call fall back to the work queue).
-bool set_graph_edge_cap(int cap);
-
-// The work-queue algorithm; with `gate` set it only runs if *gate (device
-// memory, read when the kernel starts) has one of `gate_bits`.
-hipError_t run_fill_gated(float* state, const float* src, const uint8_t* bc, int64_t nx,
-                          int64_t ny, int mode, TileShape ts, hipStream_t stream,
-                          RunStats* stats, const unsigned* gate, unsigned gate_bits);
+                    int mode, TileShape ts, hipStream_t stream, RunStats* stats);
 
 // fillsinks (work-queue algorithm, 128 x 128 tiles) with its queue
 // workspace in a fixed per-GPU device array instead of an allocation:

@@ -132,9 +132,12 @@ __device__ inline void fill_slot(const TileQueue& q, unsigned pos, int t) {
   unsigned long long* slot = &q.ring[pos % (unsigned)q.ntiles];
   // the slot's previous entry (pos - ntiles) has a ticket holder, since at
   // most ntiles tiles are in the ring; wait until it has emptied the slot
+  // once an error is set the poppers stop taking tickets, so the slot may
+  // never empty: the push is dropped (the grid is ending anyway)
   const unsigned long long t0 = wall_clock64();
   while (__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) {
     __builtin_amdgcn_s_sleep(1);
+    if (ld_relaxed(&q.sync->error) != 0u) return;
     if (wall_clock64() - t0 > q.wait_limit) {
       raise_error(q, kErrTimeout);
       return;

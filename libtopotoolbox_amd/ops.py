@@ -61,19 +61,6 @@ def set_tile_shape(qx: int, qy: int) -> None:
     _lib.check(_lib.lib().tt_set_tile_shape(qx, qy), "tt_set_tile_shape")
 
 
-FILL_METHODS = {"graph": 0, "queue": 1}
-
-
-def set_fill_method(method: str) -> None:
-    """Algorithm of fillsinks for later calls on this thread: "queue"
-    (default: tiles revisited until converged) or "graph" (tile label
-    graph, each tile staged twice). Both give the same bits; reconstruct
-    always uses the queue."""
-    if method not in FILL_METHODS:
-        raise ValueError(f"method must be one of {sorted(FILL_METHODS)}")
-    _lib.check(_lib.lib().tt_set_fill_method(FILL_METHODS[method]), "tt_set_fill_method")
-
-
 def fillsinks(dem: torch.Tensor, bc: torch.Tensor, out: torch.Tensor | None = None,
               stats: bool = False):
     """Filled DEM. ``bc == 1`` marks fixed cells (reference fillsinks)."""

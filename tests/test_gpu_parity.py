@@ -144,63 +144,6 @@ def test_fillsinks_pits_1024(cuda, oracle_lib):
     print("stats", st)
 
 
-@pytest.mark.parametrize("method,cap", [("queue", 2048), ("graph", 2048), ("graph", 3)])
-def test_fill_methods(cuda, oracle_lib, method, cap):
-    """Both fillsinks algorithms (tile graph, work queue) and the tile
-    graph's on-device fallback (an edge table of 3 forces it) give the
-    oracle's bits over every tile shape; reconstruct (always the work
-    queue) is unaffected by the setting."""
-    from libtopotoolbox_amd import _lib, ops
-
-    ops.set_fill_method(method)
-    _lib.check(_lib.lib().tt_set_fill_edge_cap(cap), "tt_set_fill_edge_cap")
-    try:
-        rng = np.random.default_rng(17)
-        for shape in [(1, 1), (3, 200), (129, 1), (65, 129), (200, 257)]:
-            dem = oracle_lib.grid_random(shape[0] * 7 + shape[1], shape)
-            dem[rng.uniform(size=shape) < 0.03] = np.nan
-            if shape[0] > 100:
-                dem = np.round(dem * 4) / 4  # flats and ties
-            bc = edge_bc(shape) | (rng.uniform(size=shape) < 0.01).astype(np.uint8)
-            want = oracle_lib.fillsinks_pf(dem, bc)
-            for tile in TILES:
-                out, _ = gpu_fill(dem, bc, tile)
-                assert_bitwise(out, want, f"{method} cap {cap} shape {shape} tile {tile}")
-        for name in ["fillsinks_random_s0", "fillsinks_nan_raw", "fillsinks_nan_convention",
-                     "fillsinks_flats"]:
-            g = load_golden(name)
-            out, _ = gpu_fill(g["dem"], g["bc"])
-            assert_bitwise(out, g["out"], f"{method} cap {cap} golden {name}")
-        # 1024^2 with 5% single-cell pits and interior sources: 64 tiles
-        dem = oracle_lib.grid_random(1024, (1024, 1030))
-        pits = rng.uniform(size=dem.shape) < 0.05
-        dem[pits] -= rng.uniform(0, 20, size=int(pits.sum())).astype(np.float32)
-        dem[rng.uniform(size=dem.shape) < 0.002] = np.nan
-        bc = edge_bc(dem.shape) | (rng.uniform(size=dem.shape) < 0.0005).astype(np.uint8)
-        out, _ = gpu_fill(dem, bc)
-        assert_bitwise(out, oracle_lib.fillsinks_pf(dem, bc), f"{method} cap {cap} 1024 pits")
-        mask = oracle_lib.grid_random(78, (150, 190))
-        mask[rng.uniform(size=mask.shape) < 0.03] = np.nan
-        marker = np.where(rng.uniform(size=mask.shape) < 0.05, mask + 5.0,
-                          -np.inf).astype(np.float32)
-        want, _, conv = oracle_lib.reconstruct(marker, mask)
-        assert conv
-        import libtopotoolbox_amd as tt
-
-        for tile in TILES:
-            tt.set_tile_shape(*tile)
-            out = tt.reconstruct(torch.from_numpy(marker).cuda(), torch.from_numpy(mask).cuda())
-            assert_bitwise(out.cpu().numpy(), want, f"{method} cap {cap} reconstruct {tile}")
-        # an overflow fallback is not a device error
-        assert _lib.device_error(clear=True) == 0
-    finally:
-        ops.set_fill_method("queue")
-        _lib.lib().tt_set_fill_edge_cap(2048)
-        import libtopotoolbox_amd as tt
-
-        tt.set_tile_shape(2, 2)
-
-
 def test_reconstruct(cuda, oracle_lib):
     import libtopotoolbox_amd as tt
 
