@@ -152,27 +152,10 @@ def fwd_adjoint_run(workload: str, arrays: dict, steps: int, plain_ms: float):
 
 
 def batched_inputs(op: str, count: int, seed: int = 5):
-    """512 uneven tiles: side exp(U(ln 128, ln 2048)), aspect U(0.5, 2);
-    per-tile fractal (Hurst 0.8, 0-1000 m, 5% pits) with seed 5000 + t; edge
-    bc (fillsinks) or 8-site Voronoi slopes U(0.2, 0.8) with seed 9000 + t
-    (excess topography); upstream gradient N(0, 1) with seed 7000 + t."""
-    from concurrent.futures import ThreadPoolExecutor
-
+    """The uneven-tile workload (libtopotoolbox_amd.synthetic.config5_items)."""
     from libtopotoolbox_amd import synthetic as S
 
-    shapes = S.uneven_tile_shapes(count, 128, 2048, seed)
-
-    def one(t):
-        ny, nx = shapes[t]
-        d = S.dem_pits(ny, nx, 5000 + t)
-        g = np.random.default_rng(7000 + t).standard_normal((ny, nx), dtype=np.float32)
-        aux = (S.edge_bc(ny, nx) if op == "fillsinks"
-               else S.voronoi_slopes(ny, nx, 8, 0.2, 0.8, 9000 + t))
-        return d, aux, g
-
-    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
-        res = list(ex.map(one, range(count)))
-    return shapes, [r[0] for r in res], [r[1] for r in res], [r[2] for r in res]
+    return S.config5_items(op, count, seed, workers=min(16, os.cpu_count() or 1))
 
 
 def batched_run(op: str, count: int, steps: int):

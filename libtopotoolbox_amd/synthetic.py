@@ -185,3 +185,27 @@ def dem_depressions(ny: int, nx: int, seed: int, nan_blob: bool = True) -> np.nd
 
 def sha256(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def config5_items(op: str, count: int = 512, seed: int = 5, workers: int = 16):
+    """The batched workload of uneven tiles: side exp(U(ln 128, ln 2048)),
+    aspect U(0.5, 2); per-tile fractal (Hurst 0.8, 0-1000 m, 5% pits) with
+    seed 5000 + t; edge bc (op "fillsinks") or 8-site Voronoi slopes
+    U(0.2, 0.8) with seed 9000 + t (op "excesstopography"); upstream
+    gradient N(0, 1) with seed 7000 + t. Returns (shapes, dems, aux, g)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    shapes = uneven_tile_shapes(count, 128, 2048, seed)
+
+    def one(t):
+        ny, nx = shapes[t]
+        d = dem_pits(ny, nx, 5000 + t)
+        g = np.random.default_rng(7000 + t).standard_normal((ny, nx), dtype=np.float32)
+        aux = (edge_bc(ny, nx) if op == "fillsinks"
+               else voronoi_slopes(ny, nx, 8, 0.2, 0.8, 9000 + t))
+        return d, aux, g
+
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        res = list(ex.map(one, range(count)))
+    return shapes, [r[0] for r in res], [r[1] for r in res], [r[2] for r in res]
+

@@ -257,8 +257,8 @@ def test_adjoint_4096(cuda, oracle_lib):
     want = oracle_lib.fillsinks_adjoint_f64(spill, g.astype(np.float64))
     err = np.abs(grad - want) / np.maximum(1.0, np.abs(want))
     assert err.max() <= 1e-6, err.max()
-    fd = fill_fd(dem_t, bc_t, g_t, cells[:64])
-    np.testing.assert_allclose(grad.reshape(-1)[cells[:64]], fd, rtol=1e-5, atol=1e-5)
+    fd = fill_fd(dem_t, bc_t, g_t, cells)
+    np.testing.assert_allclose(grad.reshape(-1)[cells], fd, rtol=1e-5, atol=1e-5)
 
     # excess topography: autograd vs FP64 adjoint at the GPU state, FD on crops
     d = dem_t.clone().requires_grad_(True)
