@@ -39,9 +39,10 @@ METRIC = ("fillsinks/excess-topo Gcells/s & %HBM peak, 8192² fp32; "
 HBM_PEAK_GBS = 8000.0
 N = 8192
 BYTES_PER_CELL = {"fill": 9, "fsm": 12}
-# forward + adjoint: fill 9 + spill 4 + (spill, g_out, g_dem) 12; fsm 12 +
-# (u, dem, slopes, g_out) 16 + (g_dem, g_slopes) 8
-BYTES_FWD_ADJ = {"fill": 25, "fsm": 36}
+# forward + adjoint, compulsory bytes per cell: fill dem 4 + bc 1 + output 4 +
+# g_out 4 + g_dem 4 (the spill index kept out of the count, as if on chip);
+# fsm dem, slopes, output, g_out, g_dem, g_slopes, 4 each
+BYTES_FWD_ADJ = {"fill": 17, "fsm": 24}
 
 
 def make_inputs(workload: str, n: int):
@@ -158,7 +159,7 @@ def batched_inputs(op: str, count: int, seed: int = 5):
     return S.config5_items(op, count, seed, workers=min(16, os.cpu_count() or 1))
 
 
-def batched_run(op: str, count: int, steps: int):
+def batched_run(op: str, count: int, steps: int, cpu: bool = True):
     """One graph-captured persistent launch over all tiles (forward +
     adjoint, autotuned tile shape and sweep cap) against the same tiles run
     one at a time through the plain single-grid entry points."""
@@ -176,6 +177,7 @@ def batched_run(op: str, count: int, steps: int):
         b.load(dems, bc=aux, g_out=gs)
     else:
         b.load(dems, slopes=aux, g_out=gs)
+    cpu_res = batched_cpu_baseline(op, dems, aux, min(16, os.cpu_count() or 1)) if cpu else None
     del dems, aux, gs
     tune = autotune.tune(b, reps=2, use_cache=False)
     graph = b.capture(max_local=tune["max_local"])
@@ -243,6 +245,7 @@ def batched_run(op: str, count: int, steps: int):
         "one_at_a_time_ms": round(seq_ms, 3), "one_at_a_time_forward_ms": round(seq_fwd_ms, 3),
         "one_at_a_time_gcells_per_s": round(cells / (seq_ms * 1e-3) / 1e9, 4),
         "speedup_vs_one_at_a_time": round(seq_ms / ms, 3),
+        "cpu_baseline": cpu_res,
         "roofline": {"bound": "hbm", "bytes_per_cell": bytes_cell,
                      "achieved": round(bytes_cell * cells / (launch_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -338,45 +341,226 @@ def roofline(workload: str, cells: int, launch_ms: float, traffic):
 
 def measured_traffic(workload: str):
     """HBM bytes per launch from the committed rocprofv3 counter summary
-    (profiles/traffic.json), or None when none matches this workload."""
+    (profiles/traffic.json, written by `bench.py --traffic`), only when it
+    was measured from the kernel sources present now (its source stamp
+    equals the library's); otherwise None: a figure from other kernels
+    would be stale."""
+    from libtopotoolbox_amd.build import source_hash
+
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(workload, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+    if t.get("source_hash") != source_hash():
+        return None
+    return t.get(workload, {}).get("bytes_per_launch")
 
 
-def cpu_baseline(workload: str, arrays: dict, crop: int):
-    """Reference path on one host core over the top-left crop x crop cells."""
+def host_info() -> dict:
+    """CPU model and core counts of the host the baseline runs on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+class pinned_core:
+    """Pins this process to one core (the first it may use) for the block:
+    the reference path is serial, and one core makes the figure repeatable."""
+
+    def __enter__(self):
+        try:
+            self.prev = os.sched_getaffinity(0)
+            self.core = min(self.prev)
+            os.sched_setaffinity(0, {self.core})
+        except (AttributeError, OSError):
+            self.prev, self.core = None, None
+        return self
+
+    def __exit__(self, *exc):
+        if self.prev is not None:
+            os.sched_setaffinity(0, self.prev)
+
+
+def cpu_baseline(workload: str, arrays: dict):
+    """The reference path on one pinned host core over the whole workload
+    (the same 8192 x 8192 inputs the GPU ran), with the iteration count of
+    the restatement (validated against the reference: same sweeps)."""
     import oracle
 
-    sub = {k: np.ascontiguousarray(v[:crop, :crop]) for k, v in arrays.items()}
     kind = "reference" if oracle.ref() is not None else "port"
-    cells = crop * crop
-    if workload == "fill":
-        from libtopotoolbox_amd.synthetic import edge_bc
-
-        bc = sub["bc"] | edge_bc(crop, crop)
-        t0 = time.perf_counter()
-        if kind == "reference":
-            oracle.ref_fillsinks(sub["dem"], bc)
+    dem = np.ascontiguousarray(arrays["dem"])
+    cells = dem.size
+    with pinned_core() as pin:
+        if workload == "fill":
+            bc = np.ascontiguousarray(arrays["bc"])
+            t0 = time.perf_counter()
+            if kind == "reference":
+                oracle.ref_fillsinks(dem, bc)
+            else:
+                oracle.fillsinks(dem, bc)
+            dt = time.perf_counter() - t0
+            _, iters, conv = oracle.fillsinks(dem, bc)
+            sample = (f"fillsinks on the whole {dem.shape[1]}x{dem.shape[0]} workload, one call; "
+                      f"reconstruct iterations {iters} (cap 1000, converged {conv})")
         else:
-            oracle.fillsinks(sub["dem"], bc)
-        dt = time.perf_counter() - t0
-        sample = f"fillsinks on the top-left {crop}x{crop} crop (edge bc), one call"
-    else:
-        t0 = time.perf_counter()
-        if kind == "reference":
-            oracle.ref_fsm2d(sub["dem"], sub["slopes"], 30.0)
-        else:
-            oracle.fsm2d_f32(sub["dem"], sub["slopes"], 30.0)
-        dt = time.perf_counter() - t0
-        sample = f"excesstopography_fsm2d on the top-left {crop}x{crop} crop, one call"
+            slopes = np.ascontiguousarray(arrays["slopes"])
+            t0 = time.perf_counter()
+            if kind == "reference":
+                oracle.ref_fsm2d(dem, slopes, 30.0)
+            else:
+                oracle.fsm2d_f32(dem, slopes, 30.0)
+            dt = time.perf_counter() - t0
+            _, iters = oracle.fsm2d_f32(dem, slopes, 30.0)
+            sample = (f"excesstopography_fsm2d on the whole {dem.shape[1]}x{dem.shape[0]} "
+                      f"workload, one call; {iters} iterations of the 4 sweeps")
+        core = pin.core
     return {"value": round(cells / dt / 1e9, 5), "unit": "Gcells/s", "cores": 1, "kind": kind,
-            "sample": sample, "seconds": round(dt, 2),
-            "host": platform.processor() or platform.machine()}
+            "sample": sample, "seconds": round(dt, 2), "pinned_core": core, **host_info()}
+
+
+def batched_cpu_baseline(op: str, dems, aux, workers: int):
+    """The reference's forward path over all tiles of the batch, one tile
+    per task on `workers` host threads (ctypes calls release the GIL). The
+    reference has no adjoint, so this is forward only."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+
+    if oracle.ref() is None:
+        return None
+    if op == "fillsinks":
+        fn = lambda k: oracle.ref_fillsinks(dems[k], aux[k])  # noqa: E731
+    else:
+        fn = lambda k: oracle.ref_fsm2d(dems[k], aux[k], 30.0)  # noqa: E731
+    cells = sum(d.size for d in dems)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(fn, range(len(dems))))
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt / 1e9, 5), "unit": "Gcells/s", "cores": workers,
+            "kind": "reference", "seconds": round(dt, 2),
+            "sample": f"forward only (the reference has no adjoint) over all {len(dems)} "
+                      f"tiles, one tile per task on {workers} threads", **host_info()}
+
+
+TRAFFIC_KERNELS = {"fill": "fill_persistent", "fsm": "fsm_persistent",
+                   "copy": "copyBuffer"}
+
+
+def _pmc_pass(counter: str, tag: str, bench_args: list) -> dict:
+    """One rocprofv3 counter pass over a child `python bench.py ...` (the
+    parent never touches the GPU); returns {kernel name: [KiB per
+    dispatch]} from the CSV it writes under gpurun_out/."""
+    import csv
+    import glob
+    import subprocess
+
+    out = os.path.join(ROOT, "gpurun_out", f"pmc_{tag}_{counter}")
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "p", "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), *bench_args]
+    print("[traffic] " + " ".join(cmd), file=sys.stderr, flush=True)
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = subprocess.run(cmd, env=env, timeout=240, capture_output=True, text=True)
+    if res.returncode != 0 and not glob.glob(os.path.join(out, "**", "*counter_collection.csv"),
+                                             recursive=True):
+        raise RuntimeError(f"rocprofv3 pass failed ({res.returncode}): {res.stderr[-2000:]}")
+    vals = {}
+    for path in glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def measure_traffic(n: int):
+    """HBM bytes per launch of the fill and excess-topography kernels at
+    n x n: FETCH_SIZE and WRITE_SIZE in passes of their own, scaled by what
+    the same counters read for a plain 256 MiB device copy."""
+    import subprocess
+
+    from libtopotoolbox_amd.build import source_hash
+
+    def pick(vals, key):
+        rows = [v for k, v in vals.items() if TRAFFIC_KERNELS[key] in k]
+        if not rows:
+            raise RuntimeError(f"no {TRAFFIC_KERNELS[key]} dispatch in the counter output")
+        flat = [x for r in rows for x in r]
+        return flat
+
+    copy_kib = 256 * 1024
+    cal = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        vals = _pmc_pass(counter, "copy", ["--copy-calibration", "256"])
+        cal[counter] = float(np.mean(pick(vals, "copy")))
+    scale = {c: copy_kib / v for c, v in cal.items()}
+    result = {"calibration": {"copy_kib": copy_kib, "fetch_kib": round(cal["FETCH_SIZE"], 1),
+                              "write_kib": round(cal["WRITE_SIZE"], 1),
+                              "fetch_scale": round(scale["FETCH_SIZE"], 4),
+                              "write_scale": round(scale["WRITE_SIZE"], 4)},
+              "source_hash": source_hash(), "size": n}
+    try:
+        result["commit"] = subprocess.run(["git", "rev-parse", "--short=12", "HEAD"], cwd=ROOT,
+                                          capture_output=True, text=True).stdout.strip() or None
+    except OSError:
+        result["commit"] = None
+    for wl in ("fill", "fsm"):
+        args = ["--workload", wl, "--size", str(n), "--steps", "3", "--warmup", "0", "--no-cpu",
+                "--no-extra", "--no-adjoint", "--no-batched", "--no-stencils"]
+        raw = {c: pick(_pmc_pass(c, wl, args), wl) for c in ("FETCH_SIZE", "WRITE_SIZE")}
+        rd = float(np.mean(raw["FETCH_SIZE"])) * 1024 * scale["FETCH_SIZE"]
+        wr = float(np.mean(raw["WRITE_SIZE"])) * 1024 * scale["WRITE_SIZE"]
+        algo = BYTES_PER_CELL[wl] * n * n
+        result[wl] = {"kernel": TRAFFIC_KERNELS[wl],
+                      "fetch_kib_raw": [round(x, 1) for x in raw["FETCH_SIZE"]],
+                      "write_kib_raw": [round(x, 1) for x in raw["WRITE_SIZE"]],
+                      "read_bytes": int(rd), "write_bytes": int(wr),
+                      "bytes_per_launch": int(rd + wr), "algorithmic_bytes": algo,
+                      "traffic_over_algorithmic": round((rd + wr) / algo, 2)}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(result, f, indent=1)
+    lines = [f"# HBM traffic per launch: fill_persistent and fsm_persistent, {n}x{n}", "",
+             "Measured by `python bench.py --traffic` on one MI355X (kernel sources "
+             f"{result['source_hash']}, commit {result['commit']}). It runs each counter in a "
+             "rocprofv3 pass of its own over a child process:", "", "```"]
+    lines += ["rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d gpurun_out/pmc_copy_<C> "
+              "-o p -- python bench.py --copy-calibration 256",
+              "rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d gpurun_out/pmc_<wl>_<C> "
+              f"-o p -- python bench.py --workload <fill|fsm> --size {n} --steps 3 --warmup 0 "
+              "--no-cpu --no-extra --no-adjoint --no-batched --no-stencils", "```", "",
+              "## Calibration", "",
+              f"A plain 256 MiB device copy reads FETCH_SIZE = {cal['FETCH_SIZE']:.1f} KiB and "
+              f"WRITE_SIZE = {cal['WRITE_SIZE']:.1f} KiB per dispatch against {copy_kib} KiB moved "
+              f"each way: scale factors {scale['FETCH_SIZE']:.4f} and {scale['WRITE_SIZE']:.4f}.",
+              "", "## Per launch (mean over the dispatches, scaled)", "",
+              "| kernel | FETCH_SIZE raw KiB | WRITE_SIZE raw KiB | HBM read | HBM write | total | "
+              "algorithmic | ratio |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for wl in ("fill", "fsm"):
+        r = result[wl]
+        lines.append(f"| `{r['kernel']}` ({wl}) | {np.mean(r['fetch_kib_raw']):.0f} | "
+                     f"{np.mean(r['write_kib_raw']):.0f} | {r['read_bytes'] / 1e9:.2f} GB | "
+                     f"{r['write_bytes'] / 1e9:.2f} GB | {r['bytes_per_launch'] / 1e9:.2f} GB | "
+                     f"{r['algorithmic_bytes'] / 1e9:.3f} GB ({BYTES_PER_CELL[wl]} B/cell) | "
+                     f"{r['traffic_over_algorithmic']}x |")
+    lines += ["", "`bench.py` reports `bytes_per_launch` as `roofline.traffic` only while the "
+              "library's source stamp equals the one above."]
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic_8192.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(json.dumps(result), flush=True)
 
 
 def main():
@@ -386,7 +570,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["fill", "fsm"], default="fill")
     ap.add_argument("--size", type=int, default=N)
-    ap.add_argument("--cpu-crop", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -396,10 +579,17 @@ def main():
                     help="only time the stencils next to the path (identifyflats, value "
                          "filters, gwdt_computecosts) and print their JSON")
     ap.add_argument("--batch-tiles", type=int, default=512)
+    ap.add_argument("--traffic", action="store_true",
+                    help="measure HBM bytes per launch of the fill and excess-topography "
+                         "kernels with rocprofv3 counter passes (child processes) and write "
+                         "profiles/traffic.json and profiles/hbm_traffic_8192.md")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.traffic:
+        measure_traffic(args.size)
+        return
     if args.copy_calibration:
         import torch
 
@@ -460,7 +650,8 @@ def main():
     }
     del r
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload, arrays, min(args.cpu_crop, n))
+        line["cpu_baseline"] = cpu_baseline(args.workload, arrays)
+        print(f"[bench] cpu baseline: {line['cpu_baseline']}", file=sys.stderr, flush=True)
     extra = {}
     # the host-pointer drop-in (reference signature): PCIe copies included
     from libtopotoolbox_amd import ops
@@ -495,7 +686,7 @@ def main():
               "kernel_stats": {k: r2["stats"][k] for k in ("requeues", "visits", "local_iters",
                                                            "max_local", "grid")}}
         if not args.no_cpu:
-            ex["cpu_baseline"] = cpu_baseline(other, a2, min(2048, n))
+            ex["cpu_baseline"] = cpu_baseline(other, a2)
         extra[other] = ex
         if not args.no_adjoint:
             extra.setdefault("fwd_adjoint", {})[other] = fwd_adjoint_run(
@@ -510,7 +701,8 @@ def main():
         torch.cuda.empty_cache()
         extra["batched"] = {}
         for op in ("fillsinks", "excesstopography"):
-            extra["batched"][op] = batched_run(op, args.batch_tiles, max(3, args.steps // 4))
+            extra["batched"][op] = batched_run(op, args.batch_tiles, max(3, args.steps // 4),
+                                               cpu=not args.no_cpu)
             print(f"[bench] batched {op}: {json.dumps(extra['batched'][op])}", file=sys.stderr,
                   flush=True)
     if extra:
