@@ -106,6 +106,23 @@ TOPOTOOLBOX_API
 void excesstopography_fmm2d(float *excess, ptrdiff_t *heap, ptrdiff_t *back, float *dem,
                             float *threshold_slopes, float cellsize, ptrdiff_t dims[2]);
 
+/* Replaces excesstopography_fmm3d, reference include/topotoolbox.h:780-783
+   (src/excesstopography.c:174-262): excess topography with layered
+   lithology. lithstack holds nlayers layer tops per cell, bottom up
+   (lithstack[(j * dims[0] + i) * nlayers + layer]); threshold_slopes one
+   slope per layer. A cell takes the Godunov proposal of the lowest layer
+   whose proposal lies below that layer's top. Solved by fast sweeping on
+   the GPU (`heap` and `back` are not used): the reference's heap order and
+   the sweeping order can reach different surfaces where the layer choice
+   switches, since the choice is not monotone in the neighbours; both keep
+   excess <= dem, and with one layer the result is that of
+   excesstopography_fsm2d. Threshold slopes must be >= 0; 1 <= nlayers <=
+   1024. */
+TOPOTOOLBOX_API
+void excesstopography_fmm3d(float *excess, ptrdiff_t *heap, ptrdiff_t *back, float *dem,
+                            float *lithstack, float *threshold_slopes, float cellsize,
+                            ptrdiff_t dims[2], ptrdiff_t nlayers);
+
 /* Replaces identifyflats, reference include/topotoolbox.h:405
    (src/identifyflats.c:32-121). output[p] & 1: flat, & 2: sill, & 4:
    presill; returns the number of flats. */
@@ -195,6 +212,12 @@ int tt_reconstruct_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t
 int tt_excesstopography_fsm2d_dev(float *excess, const float *dem, const float *slopes,
                                   float cellsize, ptrdiff_t nx, ptrdiff_t ny, void *stream,
                                   tt_stats *stats);
+
+/* excesstopography_fmm3d on device buffers (lithstack: nx * ny * nlayers
+   floats, threshold_slopes: nlayers floats, both device memory). */
+int tt_excesstopography_fmm3d_dev(float *excess, const float *dem, const float *lithstack,
+                                  const float *threshold_slopes, float cellsize, ptrdiff_t nx,
+                                  ptrdiff_t ny, ptrdiff_t nlayers, void *stream, tt_stats *stats);
 
 /* fillsinks that also returns, for every cell, the index of the cell whose
    DEM value the output copies (its spill cell; -1 for NaN and unreached

@@ -10,6 +10,8 @@ and the batched tile scheduler (batched).
 from . import _lib
 from .ops import (
     excesstopography_fmm2d_host,
+    excesstopography_fmm3d,
+    excesstopography_fmm3d_host,
     excesstopography_fsm2d,
     excesstopography_fsm2d_host,
     fillsinks,
@@ -35,6 +37,8 @@ from .ops import (
 
 __all__ = [
     "excesstopography_fmm2d_host",
+    "excesstopography_fmm3d",
+    "excesstopography_fmm3d_host",
     "excesstopography_fsm2d",
     "excesstopography_fsm2d_host",
     "fillsinks",

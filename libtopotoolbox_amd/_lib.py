@@ -86,6 +86,11 @@ SIGNATURES = {
     "fillsinks_hybrid": (None, [_vp, _vp, _vp, _vp, _dims_t]),
     "reconstruct_hybrid": (ctypes.c_int, [_vp, _vp, _vp, _dims_t]),
     "excesstopography_fmm2d": (None, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, _dims_t]),
+    "excesstopography_fmm3d": (None, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, _dims_t,
+                                      _pd]),
+    "tt_excesstopography_fmm3d_dev": (
+        ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, _pd, _pd, _pd, _vp,
+                       ctypes.POINTER(Stats)]),
     "identifyflats": (ctypes.c_ssize_t, [_vp, _vp, _dims_t]),
     "min_filter": (None, [_vp, _vp, _vp, _dims_t, _se_dims_t]),
     "max_filter": (None, [_vp, _vp, _vp, _dims_t, _se_dims_t]),

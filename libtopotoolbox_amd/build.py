@@ -21,7 +21,7 @@ OUT_DIR = os.path.join(PKG, "_lib")
 LIB_NAME = "libtopotoolbox_amd.so"
 LIB_PATH = os.path.join(OUT_DIR, LIB_NAME)
 
-SOURCES = ["runtime.hip", "fill.hip", "fsm.hip", "adjoint.hip", "batched.hip",
+SOURCES = ["runtime.hip", "fill.hip", "fsm.hip", "fmm3d.hip", "adjoint.hip", "batched.hip",
            "stencil.hip", "gwdt.hip", "capi.hip"]
 HEADERS = ["tile_engine.h", "kernels.h", "workspace.h"]
 

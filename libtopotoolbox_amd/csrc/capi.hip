@@ -229,6 +229,25 @@ int tt_excesstopography_fsm2d_dev(float* excess, const float* dem, const float* 
   return 0;
 }
 
+int tt_excesstopography_fmm3d_dev(float* excess, const float* dem, const float* lithstack,
+                                  const float* threshold_slopes, float cellsize, ptrdiff_t nx,
+                                  ptrdiff_t ny, ptrdiff_t nlayers, void* stream, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0 || nlayers < 1 || nlayers > 1024) {
+    tt::set_error("tt_excesstopography_fmm3d_dev: dims must be >= 0 and 1 <= nlayers <= 1024");
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fmm3d(excess, dem, lithstack, threshold_slopes, cellsize, nx, ny,
+                               nlayers, (hipStream_t)stream, stats ? &st : nullptr);
+  if (e != hipSuccess) return fail("tt_excesstopography_fmm3d_dev", e);
+  if (stats) {
+    to_public(st, stats);
+    if (st.error) return device_error("tt_excesstopography_fmm3d_dev", st.error);
+  }
+  return 0;
+}
+
 int tt_fillsinks_spill_dev(float* output, int32_t* spill, const float* dem, const uint8_t* bc,
                            ptrdiff_t nx, ptrdiff_t ny, void* stream, tt_stats* stats) {
   tt::set_error("%s", "");
@@ -446,6 +465,35 @@ void excesstopography_fmm2d(float* excess, ptrdiff_t* heap, ptrdiff_t* back, flo
   (void)heap;
   (void)back;
   excesstopography_fsm2d(excess, dem, threshold_slopes, cellsize, dims);
+}
+
+void excesstopography_fmm3d(float* excess, ptrdiff_t* heap, ptrdiff_t* back, float* dem,
+                            float* lithstack, float* threshold_slopes, float cellsize,
+                            ptrdiff_t dims[2], ptrdiff_t nlayers) {
+  (void)heap;
+  (void)back;
+  const char* fn = "excesstopography_fmm3d";
+  tt::set_error("%s", "");
+  if (!dims_ok(dims, fn)) return report(fn);
+  if (nlayers < 1 || nlayers > 1024) {
+    tt::set_error("%s: nlayers must be in 1..1024", fn);
+    return report(fn);
+  }
+  const size_t n = (size_t)dims[0] * (size_t)dims[1];
+  if (n == 0) return;
+  const size_t nl = (size_t)nlayers;
+  HostStage st;
+  const size_t sz[4] = {n * 4, n * 4, n * nl * 4, nl * 4};
+  TT_HOST_TRY(fn, st.acquire(sz, 4));
+  TT_HOST_TRY(fn, hipMemcpy(st[1], dem, n * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[2], lithstack, n * nl * 4, hipMemcpyHostToDevice));
+  TT_HOST_TRY(fn, hipMemcpy(st[3], threshold_slopes, nl * 4, hipMemcpyHostToDevice));
+  if (tt_excesstopography_fmm3d_dev((float*)st[0], (const float*)st[1], (const float*)st[2],
+                                    (const float*)st[3], cellsize, dims[0], dims[1], nlayers,
+                                    nullptr, nullptr) != 0)
+    return report(fn);
+  TT_HOST_TRY(fn, hipMemcpy(excess, st[0], n * 4, hipMemcpyDeviceToHost));
+  if (sticky_check(fn) != 0) report(fn);
 }
 
 ptrdiff_t identifyflats(int32_t* output, float* dem, ptrdiff_t dims[2]) {

@@ -36,12 +36,22 @@ struct FsmView {
   long long nx, ny;
 };
 
+// Proposal of tile cell (r, c) from its x-neighbour minimum u1 and
+// y-neighbour minimum u2: the Godunov update with the cell's fi from LDS.
+template <class G>
+struct FsmProp {
+  const float* sfi;
+  __device__ __forceinline__ float operator()(int r, int c, float u1, float u2) const {
+    return godunov(u1, u2, sfi[G::at(r, c)]);
+  }
+};
+
 // Fixed-point test of a wavefront's quadrant: true (in some lane) if a cell
 // would accept its Godunov proposal. Lanes are columns walking the rows
 // (rows above / below in registers, side columns by DPP); the steps are
 // independent.
-template <class G>
-__device__ __forceinline__ bool fsm_violation(const float* su, const float* sfi, int R0, int C0,
+template <class G, class Prop>
+__device__ __forceinline__ bool fsm_violation(const float* su, const Prop& prop, int R0, int C0,
                                               int lane) {
   const int C = C0 + lane;
   const int S = (lane == 0) ? C - 1 : C + 1;
@@ -51,10 +61,9 @@ __device__ __forceinline__ bool fsm_violation(const float* su, const float* sfi,
   for (int r = 0; r < 64; r++) {
     const float dn = su[G::at(R0 + r + 1, C)];
     const float edge = su[G::at(R0 + r, S)];
-    const float fi = sfi[G::at(R0 + r, C)];
     const float u1 = fminf(lane_from_above(mid, edge), lane_from_below(mid, edge));
     const float u2 = fminf(dn, up);
-    v |= godunov(u1, u2, fi) < mid;
+    v |= prop(R0 + r, C, u1, u2) < mid;
     up = mid;
     mid = dn;
   }
@@ -76,9 +85,9 @@ __device__ __forceinline__ bool fsm_violation(const float* su, const float* sfi,
 // proposal is applied by some sweep. Two interleaved chains of 32 steps
 // (the second starts from the old value at the split) overlap each
 // other's dependent update.
-template <class G, bool VERT, bool FWD>
-__device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su,
-                                               const float* sfi, int R0, int C0, int lane) {
+template <class G, bool VERT, bool FWD, class Prop>
+__device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su, const Prop& prop,
+                                               int R0, int C0, int lane) {
   constexpr int step = FWD ? 1 : -1;
   const int out_k = FWD ? -1 : 64, end_k = FWD ? 64 : -1;
   float pc[2];
@@ -91,9 +100,6 @@ __device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su,
     e_first = su[G::at(R0 + rf, lane == 0 ? C0 - 1 : C0 + 64)];
     e_last = su[G::at(R0 + rl, lane == 0 ? C0 - 1 : C0 + 64)];
   }
-  auto fcell = [&](int i) {
-    return VERT ? sfi[G::at(R0 + i, C0 + lane)] : sfi[G::at(R0 + lane, C0 + i)];
-  };
   bool changed = false;
 #pragma unroll
   for (int s = 0; s < 32; s++) {
@@ -111,10 +117,11 @@ __device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su,
           __float_as_int(old), __float_as_int(cur), 0x130, 0xf, 0xf, false));
       const float across = fminf(hi, lo);
       const float along = FWD ? fminf(far, pc[c]) : fminf(pc[c], far);
-      const float prop = VERT ? godunov(across, along, fcell(i)) : godunov(along, across, fcell(i));
-      const bool upd = prop < cur;
+      const float p = VERT ? prop(R0 + i, C0 + lane, across, along)
+                           : prop(R0 + lane, C0 + i, along, across);
+      const bool upd = p < cur;
       changed |= upd;
-      const float nv = upd ? prop : cur;
+      const float nv = upd ? p : cur;
       u[i] = nv;
       pc[c] = nv;
     }
@@ -124,30 +131,30 @@ __device__ __forceinline__ bool fsm_sweep_regs(float (&u)[64], const float* su,
 
 // Relaxes the tile to its local fixed point with the register sweeps (state
 // in su before and after); returns the sweep iterations.
-template <class G>
-__device__ inline unsigned fsm_relax_regs(float* su, const float* sfi, int R0, int C0, int lane,
+template <class G, class Prop>
+__device__ inline unsigned fsm_relax_regs(float* su, const Prop& prop, int R0, int C0, int lane,
                                           unsigned max_local, int* any_out, int* changed_out) {
   float u[64];
   unsigned it = 0;
-  int any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
+  int any = __syncthreads_or(fsm_violation<G>(su, prop, R0, C0, lane) ? 1 : 0);
   int any_change = 0;
   while (any && it < max_local) {
 #pragma unroll
     for (int r = 0; r < 64; r++) u[r] = su[G::at(R0 + r, C0 + lane)];
-    bool ch = fsm_sweep_regs<G, true, true>(u, su, sfi, R0, C0, lane);
-    ch |= fsm_sweep_regs<G, true, false>(u, su, sfi, R0, C0, lane);
+    bool ch = fsm_sweep_regs<G, true, true>(u, su, prop, R0, C0, lane);
+    ch |= fsm_sweep_regs<G, true, false>(u, su, prop, R0, C0, lane);
 #pragma unroll
     for (int r = 0; r < 64; r++) su[G::at(R0 + r, C0 + lane)] = u[r];
 #pragma unroll
     for (int c = 0; c < 64; c++) u[c] = su[G::at(R0 + lane, C0 + c)];
-    ch |= fsm_sweep_regs<G, false, true>(u, su, sfi, R0, C0, lane);
-    ch |= fsm_sweep_regs<G, false, false>(u, su, sfi, R0, C0, lane);
+    ch |= fsm_sweep_regs<G, false, true>(u, su, prop, R0, C0, lane);
+    ch |= fsm_sweep_regs<G, false, false>(u, su, prop, R0, C0, lane);
 #pragma unroll
     for (int c = 0; c < 64; c++) su[G::at(R0 + lane, C0 + c)] = u[c];
     any = __syncthreads_or(ch ? 1 : 0);
     any_change |= any;
     it++;
-    if (any) any = __syncthreads_or(fsm_violation<G>(su, sfi, R0, C0, lane) ? 1 : 0);
+    if (any) any = __syncthreads_or(fsm_violation<G>(su, prop, R0, C0, lane) ? 1 : 0);
   }
   *any_out = any;
   *changed_out = any_change;
@@ -195,7 +202,8 @@ __device__ inline unsigned fsm_visit(const FsmView& v, long long x0, long long y
   save_perimeter<G>(lds.sper, su_bits);
   const unsigned long long c1 = wall_clock64();
   int any = 0, any_change = 0;
-  const unsigned it = fsm_relax_regs<G>(su, sfi, R0, C0, lane, max_local, &any, &any_change);
+  const unsigned it =
+      fsm_relax_regs<G>(su, FsmProp<G>{sfi}, R0, C0, lane, max_local, &any, &any_change);
   const unsigned long long c2 = wall_clock64();
   // the first visit initialises the output; later visits write changed
   // tiles only

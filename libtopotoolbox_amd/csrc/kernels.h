@@ -49,6 +49,13 @@ hipError_t run_fill_noalloc(float* state, const float* src, const uint8_t* bc, i
 hipError_t run_fsm(float* u, const float* dem, const float* slopes, float cellsize, int64_t nx,
                    int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
 
+// Excess topography with layered lithology (fmm3d.hip): lithstack holds
+// nlayers layer tops per cell (cell-major), slopes one threshold slope per
+// layer (device memory). 1 <= nlayers <= 1024 (hipErrorInvalidValue else).
+hipError_t run_fmm3d(float* u, const float* dem, const float* lithstack, const float* slopes,
+                     float cellsize, int64_t nx, int64_t ny, int64_t nlayers, hipStream_t stream,
+                     RunStats* stats);
+
 // fillsinks plus the spill cell of every output value (int32, -1 for NaN
 // and unreached cells); needs nx * ny <= INT32_MAX.
 hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc, int64_t nx,

@@ -384,6 +384,60 @@ def excesstopography_fmm2d_host(dem: np.ndarray, slopes: np.ndarray,
     return out
 
 
+def excesstopography_fmm3d_host(dem: np.ndarray, lithstack: np.ndarray,
+                                threshold_slopes: np.ndarray, cellsize: float) -> np.ndarray:
+    """Reference-signature excesstopography_fmm3d. ``lithstack`` has shape
+    (ny, nx, nlayers) (layer tops, bottom up, cell-major as in the
+    reference); ``threshold_slopes`` one slope per layer."""
+    dem = _host(dem, np.float32)
+    lith = np.ascontiguousarray(lithstack, dtype=np.float32)
+    s = np.ascontiguousarray(np.asarray(threshold_slopes).reshape(-1), dtype=np.float32)
+    ny, nx = dem.shape
+    if lith.shape != (ny, nx, s.size):
+        raise ValueError(f"lithstack has shape {lith.shape}, expected {(ny, nx, s.size)}")
+    out = np.empty_like(dem)
+    heap = np.empty(dem.shape, dtype=np.intp)
+    back = np.empty(dem.shape, dtype=np.intp)
+    _lib.lib().excesstopography_fmm3d(out.ctypes.data, heap.ctypes.data, back.ctypes.data,
+                                      dem.ctypes.data, lith.ctypes.data, s.ctypes.data,
+                                      float(cellsize), _lib.dims(nx, ny), s.size)
+    err = _lib.last_error()
+    if err:
+        raise _lib.TTError(err)
+    return out
+
+
+def excesstopography_fmm3d(dem: torch.Tensor, lithstack: torch.Tensor,
+                           threshold_slopes: torch.Tensor, cellsize: float,
+                           out: torch.Tensor | None = None, stats: bool = False):
+    """Layered excess topography on the GPU (reference excesstopography_fmm3d):
+    ``lithstack`` (ny, nx, nlayers) float32 layer tops, ``threshold_slopes``
+    (nlayers,) float32, both on the DEM's device."""
+    _check_grid(dem, "dem")
+    ny, nx = dem.shape
+    if (not isinstance(lithstack, torch.Tensor) or lithstack.dim() != 3
+            or lithstack.dtype != torch.float32 or not lithstack.is_contiguous()
+            or lithstack.shape[:2] != dem.shape or lithstack.device != dem.device):
+        raise ValueError("lithstack must be a contiguous float32 (ny, nx, nlayers) tensor on "
+                         "the DEM's device")
+    nl = lithstack.shape[2]
+    if (not isinstance(threshold_slopes, torch.Tensor) or threshold_slopes.numel() != nl
+            or threshold_slopes.dtype != torch.float32 or threshold_slopes.device != dem.device):
+        raise ValueError("threshold_slopes must be float32 with one slope per layer, on the "
+                         "DEM's device")
+    s = threshold_slopes.contiguous()
+    if out is None:
+        out = torch.empty_like(dem)
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_excesstopography_fmm3d_dev(out.data_ptr(), dem.data_ptr(),
+                                                      lithstack.data_ptr(), s.data_ptr(),
+                                                      float(cellsize), nx, ny, nl,
+                                                      _stream_handle(), st)
+    _lib.check(rc, "excesstopography_fmm3d")
+    return _ret(out, st)
+
+
 def identifyflats_host(dem: np.ndarray):
     """Reference-signature identifyflats: (int32 flags, number of flats)."""
     dem = _host(dem, np.float32)

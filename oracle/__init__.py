@@ -100,6 +100,9 @@ def ref():
         R.excesstopography_fmm2d.restype = None
         R.excesstopography_fmm2d.argtypes = [_f32p, _i64p, _i64p, _f32p, _f32p,
                                              ctypes.c_float, _dims_t]
+        R.excesstopography_fmm3d.restype = None
+        R.excesstopography_fmm3d.argtypes = [_f32p, _i64p, _i64p, _f32p, _f32p, _f32p,
+                                             ctypes.c_float, _dims_t, ctypes.c_ssize_t]
         R.fillsinks_hybrid.restype = None
         R.fillsinks_hybrid.argtypes = [_f32p, _i64p, _f32p, _u8p, _dims_t]
         R.pcg4d.restype = ctypes.c_float
@@ -262,6 +265,21 @@ def ref_fsm2d(dem, slopes, cellsize):
     dem = _c(dem, np.float32)
     u = np.empty_like(dem)
     R.excesstopography_fsm2d(u, dem, _c(slopes, np.float32), np.float32(cellsize), _dims(dem))
+    return u
+
+
+def ref_fmm3d(dem, lithstack, slopes, cellsize):
+    """The compiled reference's excesstopography_fmm3d; lithstack (ny, nx,
+    nlayers), slopes (nlayers,)."""
+    R = ref()
+    dem = _c(dem, np.float32)
+    lith = _c(lithstack, np.float32)
+    s = _c(np.asarray(slopes).reshape(-1), np.float32)
+    u = np.empty_like(dem)
+    heap = np.empty(dem.shape, np.int64)
+    back = np.empty(dem.shape, np.int64)
+    R.excesstopography_fmm3d(u, heap, back, dem, lith, s, np.float32(cellsize), _dims(dem),
+                             s.size)
     return u
 
 
