@@ -2,9 +2,10 @@
 // no derivatives). Both adjoints are deterministic: the same inputs give
 // the same bits whatever the schedule.
 //
-// fillsinks backward: the spill scatter in 64-bit fixed point
-// (adjoint_visit.h); contributions to one spill cell are pre-summed per
-// workgroup in an LDS hash table. Range: |sum| < 2^31 per spill cell.
+// fillsinks backward: the spill scatter in 64-bit fixed point with a scale
+// chosen per call from max |g_out| (adjoint_visit.h), so no sum overflows;
+// contributions to one spill cell are pre-summed per workgroup in an LDS
+// hash table; non-finite g_out entries propagate as in an IEEE sum.
 //
 // excesstopography_fsm2d backward: linearise, relax lambda with the
 // persistent work-queue kernel over adj_visit, finish (adjoint_visit.h
@@ -15,13 +16,27 @@
 namespace tt {
 
 // ---------------------------------------------------------------- fill backward
-__global__ void fill_bwd_init(unsigned long long* acc, const int* spill, const float* g_out,
-                              size_t n, unsigned* err) {
+// hdr[0]: max |g_out| over the finite entries (float bits), hdr[1]: 1 if
+// any entry is non-finite. Also clears the accumulator.
+__global__ void fill_bwd_prep(unsigned long long* acc, unsigned* hdr, const float* g_out,
+                              size_t n) {
+  __shared__ unsigned smax, snf;
+  if (threadIdx.x == 0) smax = snf = 0u;
+  __syncthreads();
+  unsigned m = 0u, nf = 0u;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (size_t)gridDim.x * blockDim.x) {
     const float g = g_out[q];
-    if (!isfinite(g)) atomicOr(err, 1u);
-    acc[q] = (spill[q] == (int)q) ? to_fixed(g) : 0ull;
+    m = max(m, abs_bits_if_finite(g));
+    nf |= isfinite(g) ? 0u : 1u;
+    acc[q] = 0ull;
+  }
+  if (m) atomicMax(&smax, m);
+  if (nf) atomicOr(&snf, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (smax) atomicMax(&hdr[0], smax);
+    if (snf) atomicOr(&hdr[1], 1u);
   }
 }
 
@@ -29,11 +44,15 @@ constexpr int kBwdThreads = 256;
 constexpr int kBwdChunk = 4096;
 constexpr int kBwdSlots = 1024;
 
+// Every finite g_out[p] with spill[p] >= 0 onto acc[spill[p]], pre-summed
+// per spill cell in an LDS hash table.
 __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long long* acc,
+                                                                const unsigned* hdr,
                                                                 const int* spill,
                                                                 const float* g_out, size_t n) {
   __shared__ int keys[kBwdSlots];
   __shared__ unsigned long long vals[kBwdSlots];
+  const int e = fix_exponent(hdr[0], n);
   const int tid = threadIdx.x;
   for (size_t base = (size_t)blockIdx.x * kBwdChunk; base < n;
        base += (size_t)gridDim.x * kBwdChunk) {
@@ -46,8 +65,8 @@ __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long lo
       const size_t p = base + k;
       if (p >= n) break;
       const int s = spill[p];
-      if (s < 0 || s == (int)p) continue;
-      const unsigned long long v = to_fixed(g_out[p]);
+      if (s < 0) continue;
+      const unsigned long long v = to_fixed(g_out[p], e);
       if (v == 0ull) continue;
       unsigned h = ((unsigned)s * 2654435761u) >> 22;  // 10 bits
       bool done = false;
@@ -71,10 +90,24 @@ __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long lo
   }
 }
 
-__global__ void fill_bwd_finish(float* g_dem, const unsigned long long* acc, size_t n) {
+__global__ void fill_bwd_finish(float* g_dem, const unsigned long long* acc, const unsigned* hdr,
+                                size_t n) {
+  const int e = fix_exponent(hdr[0], n);
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (size_t)gridDim.x * blockDim.x)
-    g_dem[q] = from_fixed(acc[q]);
+    g_dem[q] = from_fixed(acc[q], e);
+}
+
+// Non-finite g_out entries, only when there are any (hdr[1]).
+__global__ void fill_bwd_nonfinite(float* g_dem, const unsigned* hdr, const int* spill,
+                                   const float* g_out, size_t n) {
+  if (hdr[1] == 0u) return;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const float g = g_out[p];
+    const int s = spill[p];
+    if (!isfinite(g) && s >= 0) combine_nonfinite(g_dem, (size_t)s, g);
+  }
 }
 
 static int grid_for(size_t n, int threads) {
@@ -83,25 +116,22 @@ static int grid_for(size_t n, int threads) {
 }
 
 hipError_t run_fill_backward(float* g_dem, const int* spill, const float* g_out, int64_t nx,
-                             int64_t ny, hipStream_t stream, uint32_t* nonfinite) {
+                             int64_t ny, hipStream_t stream) {
   const size_t n = (size_t)nx * (size_t)ny;
   if (n == 0) return hipSuccess;
   void* ws = nullptr;
   hipError_t e = hipMallocAsync(&ws, n * 8 + 256, stream);
   if (e != hipSuccess) return e;
   unsigned long long* acc = (unsigned long long*)ws;
-  unsigned* err = (unsigned*)((char*)ws + n * 8);
-  zero_words<<<1, 64, 0, stream>>>(err, 64);
-  fill_bwd_init<<<grid_for(n, 256), 256, 0, stream>>>(acc, spill, g_out, n, err);
+  unsigned* hdr = (unsigned*)((char*)ws + n * 8);
+  zero_words<<<1, 64, 0, stream>>>(hdr, 64);
+  fill_bwd_prep<<<grid_for(n, 256), 256, 0, stream>>>(acc, hdr, g_out, n);
   size_t chunks = (n + kBwdChunk - 1) / kBwdChunk;
   int gs = (int)(chunks > 4096 ? 4096 : chunks);
-  fill_bwd_scatter<<<gs, kBwdThreads, 0, stream>>>(acc, spill, g_out, n);
-  fill_bwd_finish<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, acc, n);
+  fill_bwd_scatter<<<gs, kBwdThreads, 0, stream>>>(acc, hdr, spill, g_out, n);
+  fill_bwd_finish<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, acc, hdr, n);
+  fill_bwd_nonfinite<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, hdr, spill, g_out, n);
   e = hipGetLastError();
-  if (nonfinite && e == hipSuccess) {
-    e = hipMemcpyAsync(nonfinite, err, 4, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  }
   hipError_t f = hipFreeAsync(ws, stream);
   return e != hipSuccess ? e : f;
 }

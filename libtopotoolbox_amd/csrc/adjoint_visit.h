@@ -283,15 +283,57 @@ __device__ inline unsigned adj_visit(const AdjView& v, long long x0, long long y
 
 // ---------------------------------------------------------------- fill backward
 // out[p] = dem[spill[p]], so g_dem[q] = sum of g_out[p] over p with
-// spill[p] == q, accumulated in 64-bit fixed point (2^-32 resolution) where
-// addition is associative: atomics in any order give the same result.
-constexpr double kFixScale = 4294967296.0;  // 2^32
-
-__device__ __forceinline__ unsigned long long to_fixed(float g) {
-  return isfinite(g) ? (unsigned long long)llrint((double)g * kFixScale) : 0ull;
+// spill[p] == q. The finite contributions are summed in 64-bit fixed point,
+// where addition is associative: atomics in any order give the same bits.
+// The scale 2^e is chosen per call (per batch item) from max |g_out| over
+// its n cells: with max < 2^E and n < 2^B, e = 62 - E - B, so every
+// contribution is below 2^(62-B) and any sum of at most n of them stays
+// below 2^62: no overflow whatever the lakes. A contribution is rounded to
+// 2^-e = 2^(E+B-62), i.e. relative resolution 2^(B-62) of max |g_out|
+// (2^-35 on 2^26 cells). Non-finite contributions are left out of the sum
+// and combined afterwards with combine_nonfinite.
+__device__ __forceinline__ unsigned abs_bits_if_finite(float g) {
+  return isfinite(g) ? (__float_as_uint(g) & 0x7fffffffu) : 0u;
 }
-__device__ __forceinline__ float from_fixed(unsigned long long a) {
-  return (float)((double)(long long)a / kFixScale);
+
+// e for max |g| given by its bits (|g| >= 0 compares as unsigned) over n
+// cells; 0 when every g is zero.
+__device__ __forceinline__ int fix_exponent(unsigned max_abs_bits, unsigned long long n) {
+  if (max_abs_bits == 0u) return 0;
+  int E;
+  (void)frexpf(__uint_as_float(max_abs_bits), &E);  // max < 2^E
+  const int B = 64 - __clzll(n);                     // n < 2^B
+  return 62 - E - B;
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float g, int e) {
+  return isfinite(g) ? (unsigned long long)llrint(ldexp((double)g, e)) : 0ull;
+}
+__device__ __forceinline__ float from_fixed(unsigned long long a, int e) {
+  return (float)ldexp((double)(long long)a, -e);
+}
+
+// IEEE addition of a non-finite g to a sum: NaN when either is NaN or the
+// two are opposite infinities, else the infinity. Commutative and
+// associative over the classes, so CAS loops in any order give one result.
+__device__ __forceinline__ float add_nonfinite(float acc, float g) {
+  const float nan = __uint_as_float(0x7fc00000u);
+  if (isnan(acc) || isnan(g)) return nan;
+  if (isinf(acc) && acc != g) return nan;
+  return g;
+}
+
+// g_dem[q] := g_dem[q] + g for a non-finite g, after the finite sum is in.
+__device__ __forceinline__ void combine_nonfinite(float* g_dem, size_t q, float g) {
+  unsigned* w = reinterpret_cast<unsigned*>(g_dem + q);
+  unsigned old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {  // at most two changes: finite -> infinity -> NaN
+    const unsigned nb = __float_as_uint(add_nonfinite(__uint_as_float(old), g));
+    if (nb == old) return;
+    const unsigned prev = atomicCAS(w, old, nb);
+    if (prev == old) return;
+    old = prev;
+  }
 }
 
 }  // namespace tt

@@ -10,9 +10,12 @@
 //
 //   fillsinks:        0 fill levels (fill_visit, kModeFill)
 //                     1 spill keys from the levels (fill_visit, kModeSpill)
-//                     2 fixed-point accumulator := own contribution
+//                     2 accumulator := 0; the item's max |g_out| (the
+//                       fixed-point scale) and whether g_out has NaN / INF
 //                     3 scatter of g_out onto spill cells
 //                     4 g_dem from the accumulator
+//                     5 NaN / INF entries onto g_dem (queued only for
+//                       items that have any)
 //   excesstopography: 0 relaxation of u (fsm_visit)
 //                     1 linearisation (sx, sy, dfic)
 //                     2 relaxation of lambda (adj_visit)
@@ -63,6 +66,8 @@ struct BatchArgs {
   const unsigned* tile_item;
   unsigned* item_phase;
   unsigned* item_pending;
+  unsigned* item_gmax;       // fillsinks adjoint: max |g_out| (float bits)
+  unsigned* item_nonfinite;  // fillsinks adjoint: g_out has NaN / INF
   int nitems, ntiles, last_phase;
   BatchBuffers buf;
   unsigned long long* acc;  // fillsinks: spill keys, then the accumulator
@@ -79,6 +84,7 @@ struct BatchTopo {
   const unsigned* tile_item;
   unsigned* item_phase;
   unsigned* item_pending;
+  const unsigned* item_nonfinite;  // non-null: the last phase is optional
   int last_phase;
 
   __device__ int neighbour(int t, int k) const {
@@ -104,6 +110,8 @@ struct BatchTopo {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const unsigned ph = ld_relaxed(&item_phase[b]);
     if ((int)ph >= last_phase) return kNoExpand;
+    if (item_nonfinite && (int)ph == last_phase - 1 && ld_relaxed(&item_nonfinite[b]) == 0u)
+      return kNoExpand;
     const unsigned n = (unsigned)items[b].ntiles;
     __hip_atomic_store(&item_phase[b], ph + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&item_pending[b], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -132,10 +140,11 @@ struct BatchTopo {
   }
 };
 
-// Scatter of g_out onto the spill cells of one tile's cells, pre-summed
-// per spill cell in an LDS hash table (fixed point: order-independent).
+// Scatter of the finite g_out of one tile's cells onto their spill cells,
+// pre-summed per spill cell in an LDS hash table (fixed point with scale
+// 2^e: order-independent).
 template <class G>
-__device__ void scatter_tile(unsigned long long* acc, const int* spill, const float* g,
+__device__ void scatter_tile(unsigned long long* acc, const int* spill, const float* g, int e,
                              long long nx, long long ny, long long x0, long long y0,
                              int* keys, unsigned long long* vals) {
   constexpr int kSlots = 1024;
@@ -150,8 +159,8 @@ __device__ void scatter_tile(unsigned long long* acc, const int* spill, const fl
     if (gx >= nx || gy >= ny) continue;
     const size_t p = (size_t)gy * nx + gx;
     const int s = spill[p];
-    if (s < 0 || s == (int)p) continue;
-    const unsigned long long v = to_fixed(g[p]);
+    if (s < 0) continue;
+    const unsigned long long v = to_fixed(g[p], e);
     if (v == 0ull) continue;
     unsigned h = ((unsigned)s * 2654435761u) >> 22;
     bool done = false;
@@ -174,7 +183,9 @@ __device__ void scatter_tile(unsigned long long* acc, const int* spill, const fl
 template <class G, int OP>
 __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
   __shared__ BatchLds<G, OP> lds;
-  const BatchTopo topo{a.items, a.tile_item, a.item_phase, a.item_pending, a.last_phase};
+  __shared__ unsigned tile_gmax, tile_nonfinite;
+  const BatchTopo topo{a.items, a.tile_item, a.item_phase, a.item_pending, a.item_nonfinite,
+                       a.last_phase};
   run_queue(a.q, topo, [&](int t, bool first, unsigned nvis) -> unsigned {
     const unsigned b = a.tile_item[t];
     const BatchItemDev it = a.items[b];
@@ -203,13 +214,37 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       }
       const int* spill = a.buf.spill + off;
       const float* g = a.buf.g_out + off;
+      const unsigned long long ncell = (unsigned long long)nx * (unsigned long long)ny;
+      float* g_dem = a.buf.g_dem + off;
       if (ph == 2) {
-        for_cells([&](size_t p) { acc[p] = (spill[p] == (int)p) ? to_fixed(g[p]) : 0ull; });
+        if (threadIdx.x == 0) tile_gmax = tile_nonfinite = 0u;
+        __syncthreads();
+        unsigned m = 0u, nf = 0u;
+        for_cells([&](size_t p) {
+          const float gp = g[p];
+          m = max(m, abs_bits_if_finite(gp));
+          nf |= isfinite(gp) ? 0u : 1u;
+          acc[p] = 0ull;
+        });
+        if (m) atomicMax(&tile_gmax, m);
+        if (nf) atomicOr(&tile_nonfinite, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          if (tile_gmax) atomicMax(&a.item_gmax[b], tile_gmax);
+          if (tile_nonfinite) atomicOr(&a.item_nonfinite[b], 1u);
+        }
       } else if (ph == 3) {
-        scatter_tile<G>(acc, spill, g, nx, ny, x0, y0, lds.hash.keys, lds.hash.vals);
+        const int e = fix_exponent(ld_relaxed(&a.item_gmax[b]), ncell);
+        scatter_tile<G>(acc, spill, g, e, nx, ny, x0, y0, lds.hash.keys, lds.hash.vals);
+      } else if (ph == 4) {
+        const int e = fix_exponent(ld_relaxed(&a.item_gmax[b]), ncell);
+        for_cells([&](size_t p) { g_dem[p] = from_fixed(acc[p], e); });
       } else {
-        float* g_dem = a.buf.g_dem + off;
-        for_cells([&](size_t p) { g_dem[p] = from_fixed(acc[p]); });
+        for_cells([&](size_t p) {
+          const float gp = g[p];
+          const int s = spill[p];
+          if (!isfinite(gp) && s >= 0) combine_nonfinite(g_dem, (size_t)s, gp);
+        });
       }
       return 0u;
     } else {
@@ -239,7 +274,8 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
 }
 
 __global__ void batch_init(TileQueue q, const BatchItemDev* items, int nitems,
-                           unsigned* item_phase, unsigned* item_pending) {
+                           unsigned* item_phase, unsigned* item_pending, unsigned* item_gmax,
+                           unsigned* item_nonfinite) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     unsigned* w = reinterpret_cast<unsigned*>(q.sync);
@@ -256,6 +292,8 @@ __global__ void batch_init(TileQueue q, const BatchItemDev* items, int nitems,
   for (int b = i; b < nitems; b += stride) {
     item_phase[b] = 0u;
     item_pending[b] = (unsigned)items[b].ntiles;
+    item_gmax[b] = 0u;
+    item_nonfinite[b] = 0u;
   }
 }
 
@@ -330,12 +368,13 @@ hipError_t batch_plan(int op, int nitems, const BatchItem* items, TileShape ts, 
   h.tile_item_off = align256(h.items_off + sizeof(BatchItemDev) * nitems);
   h.bytes = h.tile_item_off + sizeof(unsigned) * ntiles;
   // workspace: control block + ring (2 words per tile) + state + visited,
-  // item phase and pending counts, then per-cell scratch (fillsinks: the
+  // item phase, pending count, max |g_out| and non-finite flag, then
+  // per-cell scratch (fillsinks: the
   // 64-bit keys, reused as the adjoint accumulator; excess topography:
   // sx, sy, dfic, lambda)
   h.ws_queue = 0;
   h.ws_items = align256((64 + 4 * (size_t)ntiles) * 4);
-  h.ws_scratch = align256(h.ws_items + 8 * (size_t)nitems);
+  h.ws_scratch = align256(h.ws_items + 16 * (size_t)nitems);
   h.ws_bytes = h.ws_scratch + (size_t)cells_end * (op == kBatchFill ? 8 : 16);
   *plan_bytes = h.bytes;
   *workspace_bytes = h.ws_bytes;
@@ -362,7 +401,8 @@ static hipError_t launch_batch(BatchArgs a, hipStream_t stream, RunStats* stats)
   const int grid = cached < a.ntiles ? cached : a.ntiles;
   int blocks = (a.ntiles + 255) / 256;
   if (blocks > 1024) blocks = 1024;
-  batch_init<<<blocks, 256, 0, stream>>>(a.q, a.items, a.nitems, a.item_phase, a.item_pending);
+  batch_init<<<blocks, 256, 0, stream>>>(a.q, a.items, a.nitems, a.item_phase, a.item_pending,
+                                         a.item_gmax, a.item_gmax + a.nitems);
   batch_persistent<G, OP><<<grid, G::THREADS, 0, stream>>>(a);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && stats) {
@@ -430,9 +470,11 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
   a.tile_item = reinterpret_cast<const unsigned*>(pd + h.tile_item_off);
   a.item_phase = reinterpret_cast<unsigned*>(ws + h.ws_items);
   a.item_pending = a.item_phase + h.nitems;
+  a.item_gmax = a.item_pending + h.nitems;
+  a.item_nonfinite = (h.op == kBatchFill && !fwd_only) ? a.item_gmax + h.nitems : nullptr;
   a.nitems = h.nitems;
   a.ntiles = h.ntiles;
-  a.last_phase = (h.op == kBatchFill) ? (fwd_only ? 1 : 4) : (fwd_only ? 0 : 3);
+  a.last_phase = (h.op == kBatchFill) ? (fwd_only ? 1 : 5) : (fwd_only ? 0 : 3);
   a.buf = buf;
   a.max_local = max_local ? max_local : (1u << 16);
   a.q.sync = reinterpret_cast<Sync*>(ws + h.ws_queue);

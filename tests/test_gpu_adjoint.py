@@ -2,7 +2,9 @@
 
 fillsinks: the spill index is compared exactly with the oracle's
 lexicographic Priority-Flood; the backward against the FP64 scatter
-(relative 1e-6, the fixed-point resolution is 2^-32); finite differences
+(relative 1e-6; the fixed-point scale is chosen per call, pinned by
+g_out = 1/N, a lake of ~1e6 cells with g_out ~ 1e6, g_out ~ 1e-30 and
+NaN / INF entries); finite differences
 (L = sum(g * fill), eps = 1e-3 applied in float32 and divided by the
 actual float difference) are exact because outputs copy inputs.
 
@@ -21,7 +23,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import load_golden
+from conftest import assert_bitwise, assert_scatter_close, load_golden, scatter_scale_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +113,19 @@ def test_fill_backward_vs_fp64(cuda, oracle_lib):
         # determinism: a second run gives the same bits
         again = ops.fillsinks_backward(to_gpu(spill.astype(np.int32)), to_gpu(g)).cpu().numpy()
         np.testing.assert_array_equal(again.view(np.uint32), got.view(np.uint32))
+
+
+def test_fill_backward_scale_and_nonfinite(cuda, oracle_lib):
+    from libtopotoolbox_amd import ops
+
+    for name, dem, bc, g in scatter_scale_cases(oracle_lib):
+        _, spill = oracle_lib.fillsinks_spill(dem, bc)
+        want = oracle_lib.fillsinks_adjoint_f64(spill, g.astype(np.float64))
+        sp = to_gpu(spill.astype(np.int32))
+        got = ops.fillsinks_backward(sp, to_gpu(g)).cpu().numpy()
+        assert_scatter_close(got, want, g, name)
+        again = ops.fillsinks_backward(sp, to_gpu(g)).cpu().numpy()
+        assert_bitwise(again, got, f"{name}: second run")
 
 
 def fill_fd(dem_t, bc_t, g_t, cells, eps=1e-3):

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import assert_bitwise
+from conftest import assert_bitwise, assert_scatter_close, scatter_scale_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +72,30 @@ def check_fill(b, oracle_lib, dems, bcs, gs, what):
         w = oracle_lib.fillsinks_adjoint_f64(wspill, g.astype(np.float64))
         err = np.abs(gd - w) / np.maximum(1.0, np.abs(w))
         assert err.max() <= 1e-6, (what, k, err.max())
+
+
+def test_batch_fill_adjoint_scale_and_nonfinite(cuda, oracle_lib):
+    """Per-item fixed-point scales and NaN / INF propagation: the cases of
+    very different max |g_out| share one launch, each item equals the
+    single-grid backward bit for bit and the FP64 scatter to 1e-6."""
+    from libtopotoolbox_amd import ops
+    from libtopotoolbox_amd.batched import Batch
+
+    cases = scatter_scale_cases(oracle_lib)
+    b = Batch("fillsinks", [c[1].shape for c in cases])
+    b.load([c[1] for c in cases], bc=[c[2] for c in cases], g_out=[c[3] for c in cases])
+    b.run()
+    r = b.outputs()
+    for k, (name, dem, bc, g) in enumerate(cases):
+        want, wspill = oracle_lib.fillsinks_spill(dem, bc)
+        assert_bitwise(r["output"][k].cpu().numpy(), want, f"{name}: output")
+        np.testing.assert_array_equal(r["spill"][k].cpu().numpy().astype(np.int64), wspill)
+        gd = r["g_dem"][k].cpu().numpy()
+        single = ops.fillsinks_backward(r["spill"][k].contiguous(),
+                                        torch.from_numpy(g).cuda()).cpu().numpy()
+        assert_bitwise(gd, single, f"{name}: g_dem vs single grid")
+        w = oracle_lib.fillsinks_adjoint_f64(wspill, g.astype(np.float64))
+        assert_scatter_close(gd, w, g, name)
 
 
 def fsm_rel(u, u64):
