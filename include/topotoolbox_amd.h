@@ -208,6 +208,24 @@ int tt_fillsinks_noalloc_dev(float *output, const float *dem, const uint8_t *bc,
 int tt_reconstruct_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t ny,
                        void *stream, tt_stats *stats);
 
+/* fillsinks / reconstruct with an iteration cap, the counterpart of the
+   reference's 1000-iteration cap in reconstruct (src/reconstruct.c:273):
+   the launch stops taking new work after about `max_visits` tile visits
+   (1 <= max_visits < 2^32; 128 x 128 tiles, each visited at least once).
+   The output is then a partial result: pointwise between the full result
+   and the initial marker (for fillsinks: the full fill <= output, and
+   output >= dem), and reconstruct called on it (for fillsinks: marker =
+   -output, mask = -dem, negated back) reaches the full result bit for bit.
+   The capped state differs from the reference's capped marker (another
+   update order). Waits for completion; *converged (required) = 1 when all
+   work finished before the cap. */
+int tt_fillsinks_capped_dev(float *output, const float *dem, const uint8_t *bc, ptrdiff_t nx,
+                            ptrdiff_t ny, int64_t max_visits, void *stream, int *converged,
+                            tt_stats *stats);
+int tt_reconstruct_capped_dev(float *marker, const float *mask, ptrdiff_t nx, ptrdiff_t ny,
+                              int64_t max_visits, void *stream, int *converged,
+                              tt_stats *stats);
+
 /* excesstopography_fsm2d on device buffers. */
 int tt_excesstopography_fsm2d_dev(float *excess, const float *dem, const float *slopes,
                                   float cellsize, ptrdiff_t nx, ptrdiff_t ny, void *stream,
@@ -228,9 +246,12 @@ int tt_fillsinks_spill_dev(float *output, int32_t *spill, const float *dem, cons
                            ptrdiff_t nx, ptrdiff_t ny, void *stream, tt_stats *stats);
 
 /* Adjoint of fillsinks: g_dem[q] = sum of g_out[p] over p with
-   spill[p] == q, accumulated in 64-bit fixed point (resolution 2^-32,
-   |sum| < 2^31), so the result does not depend on the order of the atomic
-   adds. Non-finite g_out entries contribute nothing. */
+   spill[p] == q. Finite entries are accumulated in 64-bit fixed point, so
+   the result does not depend on the order of the atomic adds; the scale is
+   chosen per call from max |g_out| so that no sum can overflow: with
+   max |g_out| < 2^E and nx * ny < 2^B, a term is rounded to a multiple of
+   2^(E + B - 62). NaN and INF entries propagate as in an IEEE sum
+   (opposite infinities give NaN). */
 int tt_fillsinks_backward_dev(float *g_dem, const int32_t *spill, const float *g_out,
                               ptrdiff_t nx, ptrdiff_t ny, void *stream);
 

@@ -15,6 +15,7 @@ from .ops import (
     excesstopography_fsm2d,
     excesstopography_fsm2d_host,
     fillsinks,
+    fillsinks_capped,
     fillsinks_host,
     fillsinks_hybrid_host,
     gwdt_computecosts,
@@ -30,6 +31,7 @@ from .ops import (
     min_filter_square,
     min_filter_square_host,
     reconstruct,
+    reconstruct_capped,
     reconstruct_host,
     reconstruct_hybrid_host,
     set_tile_shape,
@@ -42,6 +44,7 @@ __all__ = [
     "excesstopography_fsm2d",
     "excesstopography_fsm2d_host",
     "fillsinks",
+    "fillsinks_capped",
     "fillsinks_host",
     "fillsinks_hybrid_host",
     "gwdt_computecosts",
@@ -57,6 +60,7 @@ __all__ = [
     "min_filter_square",
     "min_filter_square_host",
     "reconstruct",
+    "reconstruct_capped",
     "reconstruct_host",
     "reconstruct_hybrid_host",
     "set_tile_shape",

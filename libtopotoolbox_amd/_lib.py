@@ -99,6 +99,12 @@ SIGNATURES = {
     "tt_fillsinks_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_fillsinks_noalloc_dev": (ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, _vp]),
     "tt_reconstruct_dev": (ctypes.c_int, [_vp, _vp, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
+    "tt_fillsinks_capped_dev": (
+        ctypes.c_int, [_vp, _vp, _vp, _pd, _pd, ctypes.c_int64, _vp,
+                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Stats)]),
+    "tt_reconstruct_capped_dev": (
+        ctypes.c_int, [_vp, _vp, _pd, _pd, ctypes.c_int64, _vp, ctypes.POINTER(ctypes.c_int),
+                       ctypes.POINTER(Stats)]),
     "tt_excesstopography_fsm2d_dev": (
         ctypes.c_int, [_vp, _vp, _vp, ctypes.c_float, _pd, _pd, _vp, ctypes.POINTER(Stats)]),
     "tt_fillsinks_spill_dev": (

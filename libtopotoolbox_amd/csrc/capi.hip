@@ -209,6 +209,45 @@ int tt_reconstruct_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t
   return 0;
 }
 
+static int fill_capped(const char* fn, float* state, const float* src, const uint8_t* bc,
+                       ptrdiff_t nx, ptrdiff_t ny, int mode, int64_t max_visits, void* stream,
+                       int* converged, tt_stats* stats) {
+  tt::set_error("%s", "");
+  if (nx < 0 || ny < 0) {
+    tt::set_error("%s: invalid dims", fn);
+    return 1;
+  }
+  if (max_visits < 1 || max_visits > (int64_t)UINT32_MAX) {
+    tt::set_error("%s: max_visits must be in [1, 2^32 - 1], got %lld", fn, (long long)max_visits);
+    return 1;
+  }
+  if (!converged) {
+    tt::set_error("%s: converged must not be NULL", fn);
+    return 1;
+  }
+  tt::RunStats st;
+  hipError_t e = tt::run_fill_capped(state, src, bc, nx, ny, mode, (unsigned)max_visits,
+                                     (hipStream_t)stream, &st, converged);
+  if (e != hipSuccess) return fail(fn, e);
+  if (stats) to_public(st, stats);
+  if (st.error) return device_error(fn, st.error);
+  return 0;
+}
+
+int tt_fillsinks_capped_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff_t nx,
+                            ptrdiff_t ny, int64_t max_visits, void* stream, int* converged,
+                            tt_stats* stats) {
+  return fill_capped("tt_fillsinks_capped_dev", output, dem, bc, nx, ny, tt::kModeFill,
+                     max_visits, stream, converged, stats);
+}
+
+int tt_reconstruct_capped_dev(float* marker, const float* mask, ptrdiff_t nx, ptrdiff_t ny,
+                              int64_t max_visits, void* stream, int* converged,
+                              tt_stats* stats) {
+  return fill_capped("tt_reconstruct_capped_dev", marker, mask, nullptr, nx, ny, tt::kModeRecon,
+                     max_visits, stream, converged, stats);
+}
+
 int tt_excesstopography_fsm2d_dev(float* excess, const float* dem, const float* slopes,
                                   float cellsize, ptrdiff_t nx, ptrdiff_t ny, void* stream,
                                   tt_stats* stats) {
@@ -274,7 +313,7 @@ int tt_fillsinks_backward_dev(float* g_dem, const int32_t* spill, const float* g
     tt::set_error("tt_fillsinks_backward_dev: dims must be >= 0 with nx * ny <= INT32_MAX");
     return 1;
   }
-  hipError_t e = tt::run_fill_backward(g_dem, spill, g_out, nx, ny, (hipStream_t)stream, nullptr);
+  hipError_t e = tt::run_fill_backward(g_dem, spill, g_out, nx, ny, (hipStream_t)stream);
   if (e != hipSuccess) return fail("tt_fillsinks_backward_dev", e);
   return 0;
 }

@@ -25,12 +25,12 @@ __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
 
 template <class G, int MODE>
 static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats,
-                              void* scratch = nullptr) {
+                              void* scratch = nullptr, unsigned max_visits = 0) {
   a.ntx = (int)((a.v.nx + G::TX - 1) / G::TX);
   a.nty = (int)((a.v.ny + G::TY - 1) / G::TY);
   a.max_local = 1u << 16;
   return launch_persistent(fill_persistent<G, MODE>, G::THREADS, a.ntx * a.nty, a, stream, stats,
-                           G::TX, G::TY, a.ntx, a.nty, scratch);
+                           G::TX, G::TY, a.ntx, a.nty, scratch, max_visits);
 }
 
 template <int MODE>
@@ -61,6 +61,27 @@ hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t n
   a.v.ny = ny;
   if (mode == kModeFill) return dispatch_fill<kModeFill>(a, ts, stream, stats);
   return dispatch_fill<kModeRecon>(a, ts, stream, stats);
+}
+
+hipError_t run_fill_capped(float* state, const float* src, const uint8_t* bc, int64_t nx,
+                           int64_t ny, int mode, unsigned max_visits, hipStream_t stream,
+                           RunStats* stats, int* converged) {
+  *converged = 1;
+  if (nx <= 0 || ny <= 0) return hipSuccess;
+  FillArgs a{};
+  a.v.state = state;
+  a.v.src = src;
+  a.v.bc = bc;
+  a.v.nx = nx;
+  a.v.ny = ny;
+  using G = TileGeom<2, 2>;
+  hipError_t e = mode == kModeFill
+                     ? launch_fill<G, kModeFill>(a, stream, stats, nullptr, max_visits)
+                     : launch_fill<G, kModeRecon>(a, stream, stats, nullptr, max_visits);
+  if (e != hipSuccess) return e;
+  *converged = stats->pending == 0 ? 1 : 0;
+  stats->error &= ~kStopVisitCap;
+  return hipSuccess;
 }
 
 hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc,

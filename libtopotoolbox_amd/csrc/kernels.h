@@ -16,6 +16,7 @@ struct RunStats {
   int64_t grid = 0;         // workgroups in the persistent grid
   int64_t tile_x = 0, tile_y = 0;
   uint32_t error = 0;       // Sync error bits
+  int64_t pending = 0;      // tiles still queued or running at the end (0: converged)
   // wall-clock ticks (100 MHz) summed over workgroups, by phase
   int64_t t_stage = 0, t_relax = 0, t_write = 0, t_wait = 0;
 };
@@ -34,6 +35,16 @@ enum FillMode : int {
 // call synchronises the stream to read the control block.
 hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
                     int mode, TileShape ts, hipStream_t stream, RunStats* stats);
+
+// run_fill (128 x 128 tiles) that stops after about max_visits (>= 1) tile
+// visits: every tile is visited at least once, and the state is a valid
+// partial result (between the fixed point and the initial marker), so a
+// later reconstruction from it reaches the same fixed point. Synchronises;
+// *converged = 1 if the launch finished all work; stats (required) without
+// the stop bit.
+hipError_t run_fill_capped(float* state, const float* src, const uint8_t* bc, int64_t nx,
+                           int64_t ny, int mode, unsigned max_visits, hipStream_t stream,
+                           RunStats* stats, int* converged);
 
 // fillsinks (work-queue algorithm, 128 x 128 tiles) with its queue
 // workspace in a fixed per-GPU device array instead of an allocation:
@@ -61,11 +72,11 @@ hipError_t run_fmm3d(float* u, const float* dem, const float* lithstack, const f
 hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_t* bc, int64_t nx,
                           int64_t ny, TileShape ts, hipStream_t stream, RunStats* stats);
 
-// g_dem[q] = sum of g_out[p] over p with spill[p] == q (deterministic).
-// `nonfinite` (host, may be null) receives 1 if g_out held NaN / INF, which
-// contribute nothing; non-null synchronises the stream.
+// g_dem[q] = sum of g_out[p] over p with spill[p] == q (deterministic;
+// fixed point scaled per call, so no overflow; NaN / INF entries propagate
+// as in an IEEE sum).
 hipError_t run_fill_backward(float* g_dem, const int* spill, const float* g_out, int64_t nx,
-                             int64_t ny, hipStream_t stream, uint32_t* nonfinite);
+                             int64_t ny, hipStream_t stream);
 
 // Adjoint of excesstopography_fsm2d at its converged output u.
 hipError_t run_fsm_backward(float* g_dem, float* g_slopes, const float* u, const float* dem,

@@ -162,6 +162,7 @@ hipError_t Workspace::finish(RunStats* st) {
       st->local_iters = (int64_t)h.local_iters;
       st->max_local = h.max_local;
       st->error = h.error;
+      st->pending = (int64_t)h.pending;
       st->t_stage = (int64_t)h.t_stage;
       st->t_relax = (int64_t)h.t_relax;
       st->t_write = (int64_t)h.t_write;

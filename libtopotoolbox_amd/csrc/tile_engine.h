@@ -42,7 +42,8 @@ struct Sync {
   unsigned int pending;     // tiles queued or being relaxed
   unsigned int error;       // kErr* bits
   unsigned int max_local;   // largest local iteration count of a visit
-  unsigned int pad[3];
+  unsigned int pops;        // tiles popped (counted only under a visit cap)
+  unsigned int pad[2];
   unsigned long long visits;       // tile visits
   unsigned long long local_iters;  // sum of local iterations over visits
   // wall-clock ticks (100 MHz) summed over workgroups, by phase
@@ -54,6 +55,9 @@ struct Sync {
 
 enum : unsigned int {
   kErrTimeout = 1u,
+  // not an error: the launch reached its visit cap and stopped early (set
+  // in the control block only, never in the sticky word)
+  kStopVisitCap = 0x100u,
 };
 
 // Default upper bound on one wait (wall-clock ticks of the 100 MHz
@@ -84,6 +88,9 @@ struct TileQueue {
   unsigned* sticky;
   // upper bound on one wait, in wall-clock ticks
   unsigned long long wait_limit;
+  // 0, or the number of tile visits after which the launch stops taking
+  // new work (a partial result); see run_queue
+  unsigned max_visits;
 };
 
 // Records error bits in the launch's control block (which stops the grid)
@@ -234,6 +241,17 @@ __device__ inline void run_queue(const TileQueue& q, const Topo& topo, Visit vis
       const int t = pop_tile(q);
       if (t >= 0)
         __hip_atomic_exchange(&q.state[t], kRunning, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // Visit cap: the pop that reaches it sets kStopVisitCap, which, like
+      // an error, makes waiting poppers return -1 and drops pushes that
+      // would wait for a ring slot. A popper whose slot is already filled
+      // still takes and visits its tile, so every tile of the initial queue
+      // is visited once and the grid drains after a bounded number of
+      // further visits. Each visit writes a complete relaxation step, so
+      // the stored state is a valid partial result.
+      if (t >= 0 && q.max_visits &&
+          __hip_atomic_fetch_add(&q.sync->pops, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+                  1u >= q.max_visits)
+        atomicOr(&q.sync->error, kStopVisitCap);
       s_tile = t;
       if (t >= 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");

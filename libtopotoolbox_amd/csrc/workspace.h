@@ -56,10 +56,13 @@ hipError_t persistent_grid(const void* kernel, int threads, int ntiles, int* gri
 // and releases an allocated workspace stream-ordered; fills `stats`
 // (synchronising) when non-null. Tiles of an ntx x nty grid start in
 // boundary-first order.
+// With max_visits > 0 the launch stops after about that many tile visits
+// (kStopVisitCap in the control block; every tile is visited at least once).
 template <class Args>
 hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a,
                              hipStream_t stream, RunStats* stats, int tile_x, int tile_y,
-                             int ntx = 0, int nty = 0, void* scratch = nullptr) {
+                             int ntx = 0, int nty = 0, void* scratch = nullptr,
+                             unsigned max_visits = 0) {
   int grid = 0;
   hipError_t e = persistent_grid((const void*)kern, threads, ntiles, &grid);
   if (e != hipSuccess) return e;
@@ -69,6 +72,7 @@ hipError_t launch_persistent(void (*kern)(Args), int threads, int ntiles, Args a
   if (e != hipSuccess) return e;
   a.q = ws.q;
   a.q.stats = stats ? 1 : 0;
+  a.q.max_visits = max_visits;
   kern<<<grid, threads, 0, stream>>>(a);
   e = hipGetLastError();
   if (stats) {

@@ -97,6 +97,48 @@ def reconstruct(marker: torch.Tensor, mask: torch.Tensor, inplace: bool = False,
     return _ret(m, st)
 
 
+def fillsinks_capped(dem: torch.Tensor, bc: torch.Tensor, max_visits: int,
+                     stats: bool = False):
+    """fillsinks that stops after about ``max_visits`` tile visits (128 x 128
+    tiles, each visited at least once), like the reference's iteration cap
+    in reconstruct. Returns (output, converged): an unconverged output is a
+    partial fill (full fill <= output, output >= dem) from which
+    ``reconstruct(-output, -dem)`` resumes to the full result. Synchronises."""
+    _check_grid(dem, "dem")
+    _check_grid(bc, "bc", torch.uint8)
+    _same_shape(dem, ("bc", bc))
+    out = torch.empty_like(dem)
+    ny, nx = dem.shape
+    conv = ctypes.c_int(0)
+    st = _stats_ptr(stats)
+    with torch.cuda.device(dem.device):
+        rc = _lib.lib().tt_fillsinks_capped_dev(out.data_ptr(), dem.data_ptr(), bc.data_ptr(),
+                                                nx, ny, int(max_visits), _stream_handle(),
+                                                ctypes.byref(conv), st)
+    _lib.check(rc, "fillsinks_capped")
+    return _ret((out, bool(conv.value)), st)
+
+
+def reconstruct_capped(marker: torch.Tensor, mask: torch.Tensor, max_visits: int,
+                       stats: bool = False):
+    """reconstruct that stops after about ``max_visits`` tile visits (the
+    reference stops at 1000 iterations). Returns (marker, converged); an
+    unconverged marker resumes with ``reconstruct``. Synchronises."""
+    _check_grid(marker, "marker")
+    _check_grid(mask, "mask")
+    _same_shape(marker, ("mask", mask))
+    m = marker.clone()
+    ny, nx = m.shape
+    conv = ctypes.c_int(0)
+    st = _stats_ptr(stats)
+    with torch.cuda.device(m.device):
+        rc = _lib.lib().tt_reconstruct_capped_dev(m.data_ptr(), mask.data_ptr(), nx, ny,
+                                                  int(max_visits), _stream_handle(),
+                                                  ctypes.byref(conv), st)
+    _lib.check(rc, "reconstruct_capped")
+    return _ret((m, bool(conv.value)), st)
+
+
 def excesstopography_fsm2d(dem: torch.Tensor, threshold_slopes: torch.Tensor, cellsize: float,
                            out: torch.Tensor | None = None, stats: bool = False):
     """Solution of the constrained eikonal equation (subtract from dem for

@@ -85,3 +85,23 @@ def test_noalloc_rejects_oversized_grid_without_gpu():
     L = _lib.lib()
     assert L.tt_fillsinks_noalloc_dev(None, None, None, 128 * 600, 128 * 600, None) == 2
     assert "at most 262128" in _lib.last_error()
+
+
+def test_capped_entries_check_arguments_without_gpu():
+    """max_visits outside [1, 2^32 - 1] and a NULL converged pointer are
+    rejected before any device call."""
+    import ctypes
+
+    from libtopotoolbox_amd import _lib
+
+    L = _lib.lib()
+    conv = ctypes.c_int(7)
+    for fn in ("tt_fillsinks_capped_dev", "tt_reconstruct_capped_dev"):
+        f = getattr(L, fn)
+        extra = (None,) if fn == "tt_fillsinks_capped_dev" else ()
+        for cap in (0, -1, 2**32):
+            assert f(None, None, *extra, 64, 64, cap, None, ctypes.byref(conv), None) == 1
+            assert "max_visits" in _lib.last_error()
+        assert f(None, None, *extra, 64, 64, 10, None, None, None) == 1
+        assert "converged" in _lib.last_error()
+        assert conv.value == 7
