@@ -161,7 +161,7 @@ def batched_inputs(op: str, count: int, seed: int = 5):
 
 def batched_run(op: str, count: int, steps: int, cpu: bool = True):
     """One graph-captured persistent launch over all tiles (forward +
-    adjoint, autotuned tile shape and sweep cap) against the same tiles run
+    adjoint, autotuned tile shape and per-size-class sweep caps) against the same tiles run
     one at a time through the plain single-grid entry points."""
     import torch
 
@@ -238,8 +238,9 @@ def batched_run(op: str, count: int, steps: int, cpu: bool = True):
     launch_ms = float(np.median(per))
     res = {
         "tiles": count, "cells": cells, "input_gen_s": round(gen_s, 1),
-        "autotune": {"tile": tune["tile"], "max_local": tune["max_local"],
-                     "table": tune["table"]},
+        "autotune": {k: tune[k] for k in ("tile", "max_local", "class_caps", "classes",
+                                          "class_edges_cells", "global", "per_class_ms",
+                                          "gain_vs_global", "table")},
         "graph_ms": round(ms, 3), "gcells_per_s": round(cells / (ms * 1e-3) / 1e9, 4),
         "forward_only_ms": round(fwd_ms, 3),
         "one_at_a_time_ms": round(seq_ms, 3), "one_at_a_time_forward_ms": round(seq_fwd_ms, 3),

@@ -333,6 +333,15 @@ typedef struct tt_batch_buffers {
 int tt_batch_plan(int op, const tt_batch_item *items, int nitems, int qx, int qy, void *plan,
                   size_t *plan_bytes, size_t *workspace_bytes);
 
+/* Per-item sweep caps (caps[b] caps the sweeps of one visit of item b's
+   tiles; 0 = the cap passed to tt_batch_run_dev), written into a host plan
+   made by tt_batch_plan, which the caller then copies to the device again.
+   An autotuner sets them per item size class (small items converge in one
+   visit, large ones pass progress to their neighbours sooner with a low
+   cap). Like the launch's cap they change the schedule, never the result.
+   Host-only. */
+int tt_batch_plan_set_sweep_caps(void *plan, int nitems, const uint32_t *caps);
+
 /* Runs a planned batch. plan_host and plan_dev hold the same plan bytes
    (host and device copies); `workspace` is device memory of at least
    workspace_bytes. max_local caps the sweeps of one tile visit (0 = 65536;

@@ -124,6 +124,8 @@ SIGNATURES = {
     "tt_batch_plan": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_size_t),
                                      ctypes.POINTER(ctypes.c_size_t)]),
+    "tt_batch_plan_set_sweep_caps": (ctypes.c_int, [_vp, ctypes.c_int,
+                                                     ctypes.POINTER(ctypes.c_uint32)]),
     "tt_batch_run_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.c_uint,
                                         ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),

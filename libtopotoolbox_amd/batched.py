@@ -83,11 +83,13 @@ class Batch:
             self.g_out.data_ptr(), self.g_dem.data_ptr(),
             self.g_slopes.data_ptr() if self.g_slopes is not None else None)
         self.tile = None
+        self.sweep_caps = None
         self.set_tile(tile)
 
     # ------------------------------------------------------------ planning
     def set_tile(self, tile) -> None:
-        """(Re)plan for another tile shape; the packed buffers stay."""
+        """(Re)plan for another tile shape; the packed buffers stay. The plan
+        and workspace are new buffers: capture again after this."""
         qx, qy = int(tile[0]), int(tile[1])
         L = _lib.lib()
         pb, wb = ctypes.c_size_t(0), ctypes.c_size_t(0)
@@ -98,9 +100,27 @@ class Batch:
         _lib.check(L.tt_batch_plan(OPS[self.op], self.items, n, qx, qy, host, ctypes.byref(pb),
                                    ctypes.byref(wb)), "tt_batch_plan")
         self._plan_host = host
-        self.plan_dev = torch.frombuffer(bytearray(host.raw), dtype=torch.uint8).to(self.device)
+        self.plan_dev = None
         self.workspace = torch.empty(wb.value, dtype=torch.uint8, device=self.device)
         self.tile = (qx, qy)
+        self.set_sweep_caps(self.sweep_caps)
+
+    def set_sweep_caps(self, caps) -> None:
+        """Per-item sweep caps (one per item, 0 = the cap given to ``run``;
+        None = all 0). They change the schedule, never the result."""
+        n = len(self.shapes)
+        arr = np.zeros(n, np.uint32) if caps is None else np.asarray(caps, dtype=np.uint32)
+        if arr.shape != (n,):
+            raise ValueError(f"sweep caps: expected {n} values, got shape {arr.shape}")
+        _lib.check(_lib.lib().tt_batch_plan_set_sweep_caps(
+            self._plan_host, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+            "tt_batch_plan_set_sweep_caps")
+        self.sweep_caps = None if caps is None else arr.copy()
+        plan = torch.frombuffer(bytearray(self._plan_host.raw), dtype=torch.uint8)
+        if self.plan_dev is None:
+            self.plan_dev = plan.to(self.device)
+        else:  # in place: a captured graph keeps reading this buffer
+            self.plan_dev.copy_(plan)
 
     # ------------------------------------------------------------ data
     def views(self, flat: torch.Tensor) -> list:

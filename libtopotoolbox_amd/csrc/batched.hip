@@ -194,6 +194,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
     const long long nx = it.nx, ny = it.ny;
     const size_t off = (size_t)it.offset;
     const unsigned ph = ld_relaxed(&a.item_phase[b]);
+    const unsigned ml = it.max_local ? it.max_local : a.max_local;
     // per-cell phases walk the tile's cells directly
     auto for_cells = [&](auto fn) {
       for (int idx = threadIdx.x; idx < G::TY * G::TX; idx += G::THREADS) {
@@ -207,9 +208,9 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
         const FillView v{a.buf.output + off, a.buf.spill + off, acc, a.buf.dem + off,
                          a.buf.bc + off, nx, ny, ph == 1};
         if (ph == 0)
-          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(),
+          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(),
                                           lds.level);
-        return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(),
+        return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(),
                                          lds.fill);
       }
       const int* spill = a.buf.spill + off;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       if (ph == 0) {
         const FsmView v{a.buf.output + off, a.buf.dem + off, a.buf.slopes + off, it.cellsize,
                         nx, ny};
-        return fsm_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds.fsm);
+        return fsm_visit<G>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.fsm);
       }
       if (ph == 1) {
         for_cells([&](size_t p) {
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       }
       if (ph == 2) {
         const AdjView v{a.lam + off, a.buf.g_out + off, a.sx + off, a.sy + off, nx, ny};
-        return adj_visit<G>(v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds.adj);
+        return adj_visit<G>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.adj);
       }
       for_cells([&](size_t p) {
         fsm_adjoint_finish_cell(p, a.buf.g_dem + off, a.buf.g_slopes + off, a.lam + off,
@@ -386,6 +387,28 @@ hipError_t batch_plan(int op, int nitems, const BatchItem* items, TileShape ts, 
   unsigned* ti = reinterpret_cast<unsigned*>(p + h.tile_item_off);
   for (int b = 0; b < nitems; b++)
     for (int k = 0; k < dev[b].ntiles; k++) ti[dev[b].tile_base + k] = (unsigned)b;
+  return hipSuccess;
+}
+
+hipError_t batch_plan_set_sweep_caps(void* plan, int nitems, const unsigned* caps) {
+  if (!plan || !caps) {
+    set_error("tt_batch_plan_set_sweep_caps: plan and caps are required");
+    return hipErrorInvalidValue;
+  }
+  BatchPlanHeader h;
+  memcpy(&h, plan, sizeof h);
+  if (h.magic != kBatchPlanMagic || h.nitems != nitems) {
+    set_error("tt_batch_plan_set_sweep_caps: not a plan of %d items made by tt_batch_plan",
+              nitems);
+    return hipErrorInvalidValue;
+  }
+  char* p = static_cast<char*>(plan) + h.items_off;
+  for (int b = 0; b < nitems; b++) {
+    BatchItemDev d;
+    memcpy(&d, p + sizeof(BatchItemDev) * b, sizeof d);
+    d.max_local = caps[b];
+    memcpy(p + sizeof(BatchItemDev) * b, &d, sizeof d);
+  }
   return hipSuccess;
 }
 

@@ -400,6 +400,12 @@ int tt_batch_plan(int op, const tt_batch_item* items, int nitems, int qx, int qy
   return e == hipSuccess ? 0 : (int)e;
 }
 
+int tt_batch_plan_set_sweep_caps(void* plan, int nitems, const uint32_t* caps) {
+  tt::set_error("%s", "");
+  hipError_t e = tt::batch_plan_set_sweep_caps(plan, nitems, caps);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
 int tt_batch_run_dev(const void* plan_host, const void* plan_dev,
                      const tt_batch_buffers* buffers, void* workspace, size_t workspace_bytes,
                      unsigned max_local, int flags, void* stream, tt_stats* stats) {

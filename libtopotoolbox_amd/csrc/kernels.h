@@ -100,6 +100,7 @@ struct BatchItemDev {
   int ntx, nty;     // tiles along x, y
   int ntiles;       // ntx * nty
   int tile_base;    // global id of the item's first tile
+  unsigned max_local;  // sweep cap of this item's visits (0: the launch's)
 };
 
 // Plan layout: header | items (BatchItemDev) | tile -> item (unsigned),
@@ -116,6 +117,10 @@ struct BatchPlanHeader {
 // Host-only: sizes (plan == null) or writes the plan.
 hipError_t batch_plan(int op, int nitems, const BatchItem* items, TileShape ts, void* plan,
                       size_t* plan_bytes, size_t* workspace_bytes);
+
+// Host-only: per-item sweep caps written into a plan (caps[b] = 0: the
+// launch's cap). The plan must be copied to the device again afterwards.
+hipError_t batch_plan_set_sweep_caps(void* plan, int nitems, const unsigned* caps);
 
 // One persistent launch (plus a small init kernel) over a planned batch;
 // no allocation, no synchronisation unless stats is non-null.

@@ -73,3 +73,28 @@ def test_batch_packing_on_host():
         b.load(dems[::-1], bc=bcs)
     b.set_tile((1, 1))
     assert b.tile == (1, 1) and b.plan_dev.dtype == torch.uint8
+
+
+def test_plan_sweep_caps_host_only():
+    """Per-item sweep caps are written into the host plan (the item records
+    change, nothing else); a plan of another item count or no caps is
+    refused."""
+    from libtopotoolbox_amd import _lib
+
+    L = _lib.lib()
+    shapes = [(64, 64), (300, 200), (17, 900)]
+    rc, raw, pb, wb = plan(1, shapes)
+    assert rc == 0
+    buf = ctypes.create_string_buffer(raw, pb)
+    caps = (ctypes.c_uint32 * 3)(7, 0, 123456)
+    assert L.tt_batch_plan_set_sweep_caps(buf, 3, caps) == 0
+    new = buf.raw[:pb]
+    diff = [i for i in range(pb) if new[i] != raw[i]]
+    assert diff, "caps left the plan unchanged"
+    for v in (7, 123456):  # each non-zero cap is stored as a little-endian word
+        assert struct.pack("<I", v) in new
+    assert L.tt_batch_plan_set_sweep_caps(buf, 3, (ctypes.c_uint32 * 3)(0, 0, 0)) == 0
+    assert buf.raw[:pb] == raw
+    assert L.tt_batch_plan_set_sweep_caps(buf, 4, caps) != 0
+    assert "not a plan of 4 items" in _lib.last_error()
+    assert L.tt_batch_plan_set_sweep_caps(buf, 3, None) != 0

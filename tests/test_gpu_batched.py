@@ -182,12 +182,24 @@ def test_batch_forward_only_and_sweep_cap(cuda, oracle_lib):
         assert_bitwise(r["output"][k].cpu().numpy(), want, f"capped item {k}")
         np.testing.assert_array_equal(r["spill"][k].cpu().numpy().astype(np.int64), wspill)
 
+    # per-item caps (1 sweep on every other item, the launch's cap of 3 on
+    # the rest), forward + adjoint: the same results
+    gs = grads(SHAPES, 9)
+    b.load(dems, bc=bcs, g_out=gs)
+    b.set_sweep_caps([1 if k % 2 else 0 for k in range(len(SHAPES))])
+    st = b.run(max_local=3, stats=True)
+    assert st["error"] == 0 and st["requeues"] > 0
+    check_fill(b, oracle_lib, dems, bcs, gs, "per-item caps")
+
     dems, slopes = fsm_items(7)
     gs = grads(SHAPES, 8)
     f = Batch("excesstopography", SHAPES, cellsize=30.0)
     f.load(dems, slopes=slopes, g_out=gs)
     assert f.run(max_local=2, stats=True)["error"] == 0
     check_fsm(f, oracle_lib, dems, slopes, gs, 30.0, "capped fsm")
+    f.set_sweep_caps([k % 3 for k in range(len(SHAPES))])
+    assert f.run(max_local=5, stats=True)["error"] == 0
+    check_fsm(f, oracle_lib, dems, slopes, gs, 30.0, "per-item capped fsm")
 
 
 def test_batch_graph_replays(cuda, oracle_lib):
@@ -254,10 +266,13 @@ def test_autotune_small_batch(cuda, oracle_lib, tmp_path, monkeypatch):
     gs = grads(SHAPES, 32)
     b = Batch("fillsinks", SHAPES)
     b.load(dems, bc=bcs, g_out=gs)
-    res = autotune.tune(b, reps=1, max_locals=[0, 4])
-    assert len(res["table"]) == 6 and not res["cached"]
+    res = autotune.tune(b, reps=1, caps=[0, 4])
+    # 3 tiles x 2 caps, then the other cap for the one size class present
+    assert len(res["table"]) == 7 and not res["cached"]
+    assert res["classes"] == {"small": len(SHAPES), "medium": 0, "large": 0}
     assert tuple(res["tile"]) == b.tile
     b.run(max_local=res["max_local"])
     check_fill(b, oracle_lib, dems, bcs, gs, "tuned")
-    again = autotune.tune(b, reps=1, max_locals=[0, 4])
+    again = autotune.tune(b, reps=1, caps=[0, 4])
     assert again["cached"] and again["tile"] == res["tile"]
+    assert again["class_caps"] == res["class_caps"]
