@@ -44,8 +44,10 @@ constexpr int kBwdThreads = 256;
 constexpr int kBwdChunk = 4096;
 constexpr int kBwdSlots = 1024;
 
-// Every finite g_out[p] with spill[p] >= 0 onto acc[spill[p]], pre-summed
-// per spill cell in an LDS hash table.
+// Every finite g_out[p] with spill[p] >= 0 and spill[p] != p onto
+// acc[spill[p]], pre-summed per spill cell in an LDS hash table (a cell
+// that is its own spill cell, about half of a DEM, adds its own term in
+// fill_bwd_finish instead: through the table, each would take a slot).
 __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long long* acc,
                                                                 const unsigned* hdr,
                                                                 const int* spill,
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long lo
       const size_t p = base + k;
       if (p >= n) break;
       const int s = spill[p];
-      if (s < 0) continue;
+      if (s < 0 || s == (int)p) continue;
       const unsigned long long v = to_fixed(g_out[p], e);
       if (v == 0ull) continue;
       unsigned h = ((unsigned)s * 2654435761u) >> 22;  // 10 bits
@@ -90,12 +92,14 @@ __global__ __launch_bounds__(kBwdThreads) void fill_bwd_scatter(unsigned long lo
   }
 }
 
+// g_dem from the accumulator plus the cell's own term when it is its own
+// spill cell (the same integer sum in another order).
 __global__ void fill_bwd_finish(float* g_dem, const unsigned long long* acc, const unsigned* hdr,
-                                size_t n) {
+                                const int* spill, const float* g_out, size_t n) {
   const int e = fix_exponent(hdr[0], n);
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
        q += (size_t)gridDim.x * blockDim.x)
-    g_dem[q] = from_fixed(acc[q], e);
+    g_dem[q] = from_fixed(acc[q] + (spill[q] == (int)q ? to_fixed(g_out[q], e) : 0ull), e);
 }
 
 // Non-finite g_out entries, only when there are any (hdr[1]).
@@ -129,7 +133,7 @@ hipError_t run_fill_backward(float* g_dem, const int* spill, const float* g_out,
   size_t chunks = (n + kBwdChunk - 1) / kBwdChunk;
   int gs = (int)(chunks > 4096 ? 4096 : chunks);
   fill_bwd_scatter<<<gs, kBwdThreads, 0, stream>>>(acc, hdr, spill, g_out, n);
-  fill_bwd_finish<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, acc, hdr, n);
+  fill_bwd_finish<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, acc, hdr, spill, g_out, n);
   fill_bwd_nonfinite<<<grid_for(n, 256), 256, 0, stream>>>(g_dem, hdr, spill, g_out, n);
   e = hipGetLastError();
   hipError_t f = hipFreeAsync(ws, stream);

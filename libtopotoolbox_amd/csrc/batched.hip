@@ -9,19 +9,22 @@
 // last tile of the current phase is released (a per-item pending count):
 //
 //   fillsinks:        0 fill levels (fill_visit, kModeFill)
-//                     1 spill keys from the levels (fill_visit, kModeSpill)
-//                     2 accumulator := 0; the item's max |g_out| (the
+//                     1 spill indices by labelling each tile's submerged
+//                       components (spill_ccl.h, tile by tile)
+//                     2 spill relaxation from the dirty tiles (fill_visit,
+//                       kModeSpill): components across tile edges, flats
+//                     3 accumulator := 0; the item's max |g_out| (the
 //                       fixed-point scale) and whether g_out has NaN / INF
-//                     3 scatter of g_out onto spill cells
-//                     4 g_dem from the accumulator
-//                     5 NaN / INF entries onto g_dem (queued only for
+//                     4 scatter of g_out onto spill cells
+//                     5 g_dem from the accumulator
+//                     6 NaN / INF entries onto g_dem (queued only for
 //                       items that have any)
 //   excesstopography: 0 relaxation of u (fsm_visit)
 //                     1 linearisation (sx, sy, dfic)
 //                     2 relaxation of lambda (adj_visit)
 //                     3 g_dem, g_slopes
 //
-// Forward-only runs stop after the forward phases (fillsinks: 1,
+// Forward-only runs stop after the forward phases (fillsinks: 2,
 // excess topography: 0). The launch allocates nothing and
 // never synchronises: the plan (item table) and the workspace come from
 // the caller, so the call can be captured in a graph.
@@ -33,8 +36,8 @@
 #include <vector>
 
 #include "adjoint_visit.h"
-#include "fill_visit.h"
 #include "fsm_visit.h"
+#include "spill_ccl.h"
 #include "workspace.h"
 
 namespace tt {
@@ -47,6 +50,7 @@ struct BatchLds<G, kBatchFill> {
   union {
     FillLds<G, kModeFill> level;
     FillLds<G, kModeSpill> fill;
+    SpillLabelLds<G::TX, G::TY> label;
     struct {
       int keys[1024];
       unsigned long long vals[1024];
@@ -70,7 +74,8 @@ struct BatchArgs {
   unsigned* item_nonfinite;  // fillsinks adjoint: g_out has NaN / INF
   int nitems, ntiles, last_phase;
   BatchBuffers buf;
-  unsigned long long* acc;  // fillsinks: spill keys, then the accumulator
+  unsigned long long* acc;  // fillsinks: the adjoint's accumulator
+  unsigned* dirty;          // fillsinks: per tile, relax on the first spill visit
   float* sx;                // excesstopography: linearisation and lambda
   float* sy;
   float* dfic;
@@ -140,9 +145,10 @@ struct BatchTopo {
   }
 };
 
-// Scatter of the finite g_out of one tile's cells onto their spill cells,
-// pre-summed per spill cell in an LDS hash table (fixed point with scale
-// 2^e: order-independent).
+// Scatter of the finite g_out of one tile's cells onto their spill cells
+// other than themselves, pre-summed per spill cell in an LDS hash table
+// (fixed point with scale 2^e: order-independent); a cell that is its own
+// spill cell adds its term when g_dem is formed.
 template <class G>
 __device__ void scatter_tile(unsigned long long* acc, const int* spill, const float* g, int e,
                              long long nx, long long ny, long long x0, long long y0,
@@ -159,7 +165,7 @@ __device__ void scatter_tile(unsigned long long* acc, const int* spill, const fl
     if (gx >= nx || gy >= ny) continue;
     const size_t p = (size_t)gy * nx + gx;
     const int s = spill[p];
-    if (s < 0) continue;
+    if (s < 0 || s == (int)p) continue;
     const unsigned long long v = to_fixed(g[p], e);
     if (v == 0ull) continue;
     unsigned h = ((unsigned)s * 2654435761u) >> 22;
@@ -204,20 +210,31 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
     };
     if constexpr (OP == kBatchFill) {
       unsigned long long* acc = a.acc + off;
-      if (ph <= 1) {
-        const FillView v{a.buf.output + off, a.buf.spill + off, acc, a.buf.dem + off,
-                         a.buf.bc + off, nx, ny, ph == 1};
-        if (ph == 0)
-          return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(),
-                                          lds.level);
-        return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(),
-                                         lds.fill);
+      const FillView v{a.buf.output + off, a.buf.spill + off, a.buf.dem + off, a.buf.bc + off,
+                       nx, ny};
+      if (ph == 0)
+        return fill_visit<G, kModeFill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.level);
+      if (ph == 1) {
+        // tile-by-tile labelling (label tile = relaxation tile): each tile
+        // writes its dirty flag
+        const SpillView sv{a.buf.output + off, a.buf.dem + off, a.buf.bc + off,
+                           a.buf.spill + off, nullptr, a.dirty + it.tile_base, nx, ny,
+                           G::TX, G::TY, it.ntx};
+        spill_label_tile<G::TX, G::TY, G::THREADS, false>(sv, x0, y0, lds.label);
+        return 0u;
+      }
+      if (ph == 2) {
+        if (spill_skip(a.dirty, t, first, &lds.fill.bits)) return 0u;
+        const unsigned bits =
+            fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.fill);
+        spill_mark(a.dirty, topo, t, bits);
+        return bits;
       }
       const int* spill = a.buf.spill + off;
       const float* g = a.buf.g_out + off;
       const unsigned long long ncell = (unsigned long long)nx * (unsigned long long)ny;
       float* g_dem = a.buf.g_dem + off;
-      if (ph == 2) {
+      if (ph == 3) {
         if (threadIdx.x == 0) tile_gmax = tile_nonfinite = 0u;
         __syncthreads();
         unsigned m = 0u, nf = 0u;
@@ -234,12 +251,14 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
           if (tile_gmax) atomicMax(&a.item_gmax[b], tile_gmax);
           if (tile_nonfinite) atomicOr(&a.item_nonfinite[b], 1u);
         }
-      } else if (ph == 3) {
-        const int e = fix_exponent(ld_relaxed(&a.item_gmax[b]), ncell);
-        scatter_tile<G>(acc, spill, g, e, nx, ny, x0, y0, lds.hash.keys, lds.hash.vals);
       } else if (ph == 4) {
         const int e = fix_exponent(ld_relaxed(&a.item_gmax[b]), ncell);
-        for_cells([&](size_t p) { g_dem[p] = from_fixed(acc[p], e); });
+        scatter_tile<G>(acc, spill, g, e, nx, ny, x0, y0, lds.hash.keys, lds.hash.vals);
+      } else if (ph == 5) {
+        const int e = fix_exponent(ld_relaxed(&a.item_gmax[b]), ncell);
+        for_cells([&](size_t p) {
+          g_dem[p] = from_fixed(acc[p] + (spill[p] == (int)p ? to_fixed(g[p], e) : 0ull), e);
+        });
       } else {
         for_cells([&](size_t p) {
           const float gp = g[p];
@@ -368,13 +387,12 @@ hipError_t batch_plan(int op, int nitems, const BatchItem* items, TileShape ts, 
   h.items_off = align256(sizeof(BatchPlanHeader));
   h.tile_item_off = align256(h.items_off + sizeof(BatchItemDev) * nitems);
   h.bytes = h.tile_item_off + sizeof(unsigned) * ntiles;
-  // workspace: control block + ring (2 words per tile) + state + visited,
-  // item phase, pending count, max |g_out| and non-finite flag, then
-  // per-cell scratch (fillsinks: the
-  // 64-bit keys, reused as the adjoint accumulator; excess topography:
-  // sx, sy, dfic, lambda)
+  // workspace: control block + ring (2 words per tile) + state + visited +
+  // dirty, item phase, pending count, max |g_out| and non-finite flag, then
+  // per-cell scratch (fillsinks: the adjoint's 64-bit accumulator; excess
+  // topography: sx, sy, dfic, lambda)
   h.ws_queue = 0;
-  h.ws_items = align256((64 + 4 * (size_t)ntiles) * 4);
+  h.ws_items = align256((64 + 5 * (size_t)ntiles) * 4);
   h.ws_scratch = align256(h.ws_items + 16 * (size_t)nitems);
   h.ws_bytes = h.ws_scratch + (size_t)cells_end * (op == kBatchFill ? 8 : 16);
   *plan_bytes = h.bytes;
@@ -497,13 +515,14 @@ hipError_t run_batch(const void* plan_host, const void* plan_dev, const BatchBuf
   a.item_nonfinite = (h.op == kBatchFill && !fwd_only) ? a.item_gmax + h.nitems : nullptr;
   a.nitems = h.nitems;
   a.ntiles = h.ntiles;
-  a.last_phase = (h.op == kBatchFill) ? (fwd_only ? 1 : 5) : (fwd_only ? 0 : 3);
+  a.last_phase = (h.op == kBatchFill) ? (fwd_only ? 2 : 6) : (fwd_only ? 0 : 3);
   a.buf = buf;
   a.max_local = max_local ? max_local : (1u << 16);
   a.q.sync = reinterpret_cast<Sync*>(ws + h.ws_queue);
   a.q.ring = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned*>(a.q.sync) + 64);
   a.q.state = reinterpret_cast<unsigned*>(a.q.ring + h.ntiles);
   a.q.visited = a.q.state + h.ntiles;
+  a.dirty = a.q.visited + h.ntiles;
   a.q.ntiles = h.ntiles;
   a.q.wait_limit = wait_limit_ticks();
   {

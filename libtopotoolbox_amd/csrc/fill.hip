@@ -1,8 +1,10 @@
 // Sink filling and grayscale reconstruction on one grid: the persistent
-// work-queue kernel over fill_visit (fill_visit.h states the semantics).
+// work-queue kernel over fill_visit (fill_visit.h states the semantics),
+// and the spill indices of the fillsinks adjoint (labelling of spill_ccl.h,
+// then the spill relaxation from the dirty tiles).
 #include <mutex>
 
-#include "fill_visit.h"
+#include "spill_ccl.h"
 #include "workspace.h"
 
 namespace tt {
@@ -12,15 +14,58 @@ struct FillArgs {
   int ntx, nty;
   TileQueue q;
   unsigned max_local;
+  unsigned* dirty;  // kModeSpill: per tile, relax on the first visit
 };
 
 template <class G, int MODE>
 __global__ __launch_bounds__(G::THREADS) void fill_persistent(FillArgs a) {
   __shared__ FillLds<G, MODE> lds;
-  run_queue(a.q, GridTopo{a.ntx, a.nty}, [&](int t, bool first, unsigned nvis) {
+  const GridTopo topo{a.ntx, a.nty};
+  run_queue(a.q, topo, [&](int t, bool first, unsigned nvis) -> unsigned {
+    if constexpr (MODE == kModeSpill)
+      if (spill_skip(a.dirty, t, first, &lds.bits)) return 0u;
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
-    return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds);
+    const unsigned bits =
+        fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds);
+    if constexpr (MODE == kModeSpill) spill_mark(a.dirty, topo, t, bits);
+    return bits;
   });
+}
+
+// ---------------------------------------------------------------- spill labelling
+constexpr int kLabelTile = 64;     // labelling tiles: 64 x 64 cells
+constexpr int kLabelThreads = 256;
+
+__global__ __launch_bounds__(kLabelThreads) void spill_label_kernel(SpillView v) {
+  __shared__ SpillLabelLds<kLabelTile, kLabelTile> lds;
+  spill_label_tile<kLabelTile, kLabelTile, kLabelThreads, true>(
+      v, (long long)blockIdx.x * kLabelTile, (long long)blockIdx.y * kLabelTile, lds);
+}
+
+__global__ __launch_bounds__(kLabelThreads) void spill_border_kernel(SpillView v) {
+  spill_border_tile<kLabelTile, kLabelTile, kLabelThreads>(
+      v, (long long)blockIdx.x * kLabelTile, (long long)blockIdx.y * kLabelTile);
+}
+
+__global__ void spill_root_kernel(SpillView v) {
+  const size_t n = (size_t)v.nx * v.ny;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    spill_root_cell(v, g);
+}
+
+__global__ void spill_link_kernel(SpillView v) {
+  const size_t n = (size_t)v.nx * v.ny;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    spill_link_cell(v, g);
+}
+
+__global__ void spill_final_kernel(SpillView v) {
+  const size_t n = (size_t)v.nx * v.ny;
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (size_t)gridDim.x * blockDim.x)
+    spill_final_cell(v, g);
 }
 
 template <class G, int MODE>
@@ -88,24 +133,56 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
                           int64_t nx, int64_t ny, TileShape ts, hipStream_t stream,
                           RunStats* stats) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
-  FillArgs a{};
-  a.v.state = out;
-  a.v.spill = spill;
-  a.v.src = dem;
-  a.v.bc = bc;
-  a.v.nx = nx;
-  a.v.ny = ny;
-  // exact levels first (float relaxation, the fast path), then the
-  // (level, index) keys from (level, no index): only spill indices move
+  // exact levels first (float relaxation, the fast path)
   hipError_t e = run_fill(out, dem, bc, nx, ny, kModeFill, TileShape{2, 2}, stream, nullptr);
   if (e != hipSuccess) return e;
-  a.v.levels = true;
-  // (level, index) keys live in one 64-bit array between visits; the
-  // index output is written alongside (the levels in out are final)
-  e = hipMallocAsync((void**)&a.v.keys, (size_t)nx * ny * 8, stream);
+  // labelling: spill := exact indices or upper bounds, dirty := relaxation
+  // tiles holding a flat cell
+  const size_t n = (size_t)nx * (size_t)ny;
+  const int rtx = 64 * ts.qx, rty = 64 * ts.qy;
+  const int rntx = (int)((nx + rtx - 1) / rtx), rnty = (int)((ny + rty - 1) / rty);
+  const size_t ntiles = (size_t)rntx * rnty;
+  const size_t cmin_bytes = (n * 4 + 255) & ~(size_t)255;
+  void* ws = nullptr;
+  e = hipMallocAsync(&ws, cmin_bytes + ntiles * 4, stream);
   if (e != hipSuccess) return e;
-  e = dispatch_fill<kModeSpill>(a, ts, stream, stats);
-  hipError_t f = hipFreeAsync(a.v.keys, stream);
+  SpillView sv{};
+  sv.F = out;
+  sv.dem = dem;
+  sv.bc = bc;
+  sv.spill = spill;
+  sv.cmin = static_cast<unsigned*>(ws);
+  sv.dirty = reinterpret_cast<unsigned*>(static_cast<char*>(ws) + cmin_bytes);
+  sv.nx = nx;
+  sv.ny = ny;
+  sv.rtx = rtx;
+  sv.rty = rty;
+  sv.rntx = rntx;
+  const dim3 tiles((unsigned)((nx + kLabelTile - 1) / kLabelTile),
+                   (unsigned)((ny + kLabelTile - 1) / kLabelTile));
+  size_t cb = (n + 255) / 256;
+  const int cells_grid = (int)(cb > 8192 ? 8192 : cb);
+  zero_words<<<(int)((ntiles + 255) / 256 > 1024 ? 1024 : (ntiles + 255) / 256), 256, 0, stream>>>(
+      sv.dirty, ntiles);
+  spill_label_kernel<<<tiles, kLabelThreads, 0, stream>>>(sv);
+  spill_border_kernel<<<tiles, kLabelThreads, 0, stream>>>(sv);
+  spill_root_kernel<<<cells_grid, 256, 0, stream>>>(sv);
+  spill_link_kernel<<<cells_grid, 256, 0, stream>>>(sv);
+  spill_final_kernel<<<cells_grid, 256, 0, stream>>>(sv);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    // relaxation of the indices the labelling left as upper bounds
+    FillArgs a{};
+    a.v.state = out;
+    a.v.spill = spill;
+    a.v.src = dem;
+    a.v.bc = bc;
+    a.v.nx = nx;
+    a.v.ny = ny;
+    a.dirty = sv.dirty;
+    e = dispatch_fill<kModeSpill>(a, ts, stream, stats);
+  }
+  hipError_t f = hipFreeAsync(ws, stream);
   return e != hipSuccess ? e : f;
 }
 

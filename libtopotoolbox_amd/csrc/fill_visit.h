@@ -22,9 +22,12 @@
 //             in 64 bits; the index half of the fixed point is the cell
 //             whose DEM value the output copies (the spill cell), which the
 //             adjoint scatters to. Unique fixed point, so deterministic.
-//             Halo reads race with neighbour write-backs, which is harmless
-//             only because every value read is a valid upper bound: hence
-//             one 64-bit key word per cell, never two arrays.
+//             The levels are final (state) and never change, so a key is
+//             staged from the level and the cell's spill index (one 32-bit
+//             word): a halo read racing with a neighbour's write-back sees
+//             an old or a new index, both valid upper bounds. The relaxation
+//             starts from the indices of spill_ccl.h's labelling (upper
+//             bounds, exact except near flat cells).
 #pragma once
 
 #include <type_traits>
@@ -39,17 +42,11 @@ enum : int { kModeSpill = 2 };
 // One grid as seen by a tile visit (spill indices are cell positions in
 // this grid).
 struct FillView {
-  float* state;
-  int* spill;                // kModeSpill only
-  unsigned long long* keys;  // kModeSpill only: the iteration state
+  float* state;       // kModeSpill: the final fill levels (read only)
+  int* spill;         // kModeSpill only: the iteration state's index halves
   const float* src;
   const uint8_t* bc;
   long long nx, ny;
-  // kModeSpill: must be true: state holds the exact fill levels F, so a
-  // cell starts at (F, no index) instead of (+INF, no index): a tighter
-  // upper bound of its key, which leaves only the index part to settle, and
-  // each cell's bound reduces to a kind byte (SpillBound)
-  bool levels;
 };
 
 // ---------------------------------------------------------------- level types
@@ -263,8 +260,8 @@ constexpr unsigned kNoNbr = 0xffffffffu;  // above every level / index: min() ig
 // ---------------------------------------------------------------- spill index sweeps
 // With the levels fixed, a key's relaxation only moves its index half, and
 // only between neighbours of equal level: a lower neighbour gives a key
-// below every bound (such surface cells sit at their own index from their
-// first visit on), a higher one a key above the cell's own. The register
+// below every bound (such surface cells sit at their own index from the
+// labelling on), a higher one a key above the cell's own. The register
 // sweeps below therefore relax 32-bit indices as the order-domain sweeps
 // relax levels, a neighbour counting only where a per-visit mask says its
 // level equals the cell's. Inert cells hold all ones and differ in level
@@ -581,129 +578,104 @@ struct FillLds {
   unsigned bits;
 };
 
-// Stages the keys and kinds of a spill tile + halo (kModeSpill). NaN and
-// outside cells become inert; on the first visit the start keys are made
-// from the levels in state (bc cells: their own index). Full-width tiles
-// of grids with nx % 4 == 0 load four cells per lane (a 16-byte raw load
-// and two 16-byte key loads); every load of a batch is issued before any
+// Stages the keys and kinds of a spill tile + halo (kModeSpill): key =
+// (level, spill index) from the levels in state and the current indices;
+// NaN and outside cells are inert. Full-width tiles of grids with
+// nx % 4 == 0 load four cells per lane (16-byte loads of the DEM, the
+// levels and the indices); every load of a batch is issued before any
 // value is used.
 template <class G>
 __device__ inline void stage_spill(const FillView& v, unsigned long long* sf, unsigned char* sd,
-                                   long long x0, long long y0, bool first) {
+                                   long long x0, long long y0) {
   using T = unsigned long long;
   // opaque per call: keeps the compiler from computing every batch's cell
   // addresses once, outside the persistent loop, and spilling them
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  auto store = [&](int r, int c, float raw, unsigned char b, float lev, T key) {
+  auto store = [&](int r, int c, float raw, float lev, int idx) {
     T f = KeyLevel::kInert;  // NaN and outside cells: never lowered, never lower
     unsigned char kind = kInertCell;
     if (raw == raw) {  // false for outside cells too
-      // every key's level half is the cell's fill level (the levels were
-      // staged as the start keys on the first visit)
-      if (first)
-        f = (b == 1) ? KeyLevel::make(raw, (unsigned)((y0 + r) * v.nx + x0 + c))
-                     : KeyLevel::make(lev, 0xffffffffu);
-      else
-        f = key;
-      kind = (unsigned)(f >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
+      f = KeyLevel::make(lev, (unsigned)idx);
+      kind = ord_bits(lev) == ord_bits(raw) ? kSurface : kSubmerged;
     }
     sf[G::at(r, c)] = f;
     sd[G::at(r, c)] = kind;
   };
-  auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
-  const bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx) && al(v.src, 15) &&
-                   (first ? (al(v.bc, 3) && al(v.state, 15)) : al(v.keys, 15));
+  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx) && al(v.src) && al(v.state) &&
+                   al(v.spill);
+  const float nan = __builtin_nanf("");
   if (vec) {
     // quads of the tile's columns 0..TX-1 over rows -1..TY, in two batches
     constexpr int Q = G::TX / 4, NV = G::ROWS * Q;
     constexpr int UV = (NV + 2 * G::THREADS - 1) / (2 * G::THREADS);
     for (int base = 0; base < NV; base += G::THREADS * UV) {
       float4 raw[UV], lev[UV];
-      ulonglong2 k0[UV], k1[UV];
-      uchar4 b[UV];
+      int4 idx[UV];
 #pragma unroll
       for (int u = 0; u < UV; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        const long long gy = y0 + idx / Q - 1;
-        const float nan = __builtin_nanf("");
+        const int k = base + u * G::THREADS + tid;
+        const long long gy = y0 + k / Q - 1;
         raw[u] = make_float4(nan, nan, nan, nan);
         lev[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        b[u] = make_uchar4(0, 0, 0, 0);
-        k0[u] = k1[u] = make_ulonglong2(0ull, 0ull);
-        if (idx < NV && gy >= 0 && gy < v.ny) {
-          const size_t g = (size_t)gy * v.nx + x0 + 4 * (idx % Q);
+        idx[u] = make_int4(-1, -1, -1, -1);
+        if (k < NV && gy >= 0 && gy < v.ny) {
+          const size_t g = (size_t)gy * v.nx + x0 + 4 * (k % Q);
           raw[u] = *reinterpret_cast<const float4*>(v.src + g);
-          if (first) {
-            b[u] = *reinterpret_cast<const uchar4*>(v.bc + g);
-            lev[u] = *reinterpret_cast<const float4*>(v.state + g);
-          } else {
-            k0[u] = *reinterpret_cast<const ulonglong2*>(v.keys + g);
-            k1[u] = *reinterpret_cast<const ulonglong2*>(v.keys + g + 2);
-          }
+          lev[u] = *reinterpret_cast<const float4*>(v.state + g);
+          idx[u] = *reinterpret_cast<const int4*>(v.spill + g);
         }
       }
 #pragma unroll
       for (int u = 0; u < UV; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        if (idx < NV) {
-          const int r = idx / Q - 1, c = 4 * (idx % Q);
-          store(r, c, raw[u].x, b[u].x, lev[u].x, k0[u].x);
-          store(r, c + 1, raw[u].y, b[u].y, lev[u].y, k0[u].y);
-          store(r, c + 2, raw[u].z, b[u].z, lev[u].z, k1[u].x);
-          store(r, c + 3, raw[u].w, b[u].w, lev[u].w, k1[u].y);
+        const int k = base + u * G::THREADS + tid;
+        if (k < NV) {
+          const int r = k / Q - 1, c = 4 * (k % Q);
+          store(r, c, raw[u].x, lev[u].x, idx[u].x);
+          store(r, c + 1, raw[u].y, lev[u].y, idx[u].y);
+          store(r, c + 2, raw[u].z, lev[u].z, idx[u].z);
+          store(r, c + 3, raw[u].w, lev[u].w, idx[u].w);
         }
       }
     }
     // halo columns -1 and TX
-    for (int idx = tid; idx < 2 * G::ROWS; idx += G::THREADS) {
-      const int r = idx % G::ROWS - 1, c = (idx < G::ROWS) ? -1 : G::TX;
+    for (int k = tid; k < 2 * G::ROWS; k += G::THREADS) {
+      const int r = k % G::ROWS - 1, c = (k < G::ROWS) ? -1 : G::TX;
       const long long gx = x0 + c, gy = y0 + r;
-      float raw = __builtin_nanf(""), lev = 0.f;
-      unsigned char b = 0;
-      T key = 0ull;
+      float raw = nan, lev = 0.f;
+      int idx = -1;
       if (gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
         const size_t g = (size_t)gy * v.nx + gx;
         raw = v.src[g];
-        if (first) {
-          b = v.bc[g];
-          lev = v.state[g];
-        } else {
-          key = v.keys[g];
-        }
+        lev = v.state[g];
+        idx = v.spill[g];
       }
-      store(r, c, raw, b, lev, key);
+      store(r, c, raw, lev, idx);
     }
   } else {
     constexpr int E = G::ROWS * (G::TX + 2), U = 8;
     for (int base = 0; base < E; base += G::THREADS * U) {
       float raw[U], lev[U];
-      T key[U];
-      unsigned char b[U];
+      int idx[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        const long long gx = x0 + idx % (G::TX + 2) - 1, gy = y0 + idx / (G::TX + 2) - 1;
-        raw[u] = __builtin_nanf("");
+        const int k = base + u * G::THREADS + tid;
+        const long long gx = x0 + k % (G::TX + 2) - 1, gy = y0 + k / (G::TX + 2) - 1;
+        raw[u] = nan;
         lev[u] = 0.f;
-        b[u] = 0;
-        key[u] = 0ull;
-        if (idx < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
+        idx[u] = -1;
+        if (k < E && gx >= 0 && gx < v.nx && gy >= 0 && gy < v.ny) {
           const size_t g = (size_t)gy * v.nx + gx;
           raw[u] = v.src[g];
-          if (first) {
-            b[u] = v.bc[g];
-            lev[u] = v.state[g];
-          } else {
-            key[u] = v.keys[g];
-          }
+          lev[u] = v.state[g];
+          idx[u] = v.spill[g];
         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int idx = base + u * G::THREADS + tid;
-        if (idx < E)
-          store(idx / (G::TX + 2) - 1, idx % (G::TX + 2) - 1, raw[u], b[u], lev[u], key[u]);
+        const int k = base + u * G::THREADS + tid;
+        if (k < E) store(k / (G::TX + 2) - 1, k % (G::TX + 2) - 1, raw[u], lev[u], idx[u]);
       }
     }
   }
@@ -833,83 +805,6 @@ __device__ inline void stage_levels(const FillView& v, unsigned* sf, unsigned* s
   }
 }
 
-// Writes a relaxed tile back (cells inside the grid). Inert cells (NaN
-// input) take the input bits; kModeSpill writes the keys and the spill
-// indices and leaves the levels, which never change, as they are (one
-// aligned store per key: a neighbour reading its halo concurrently sees
-// the old or the new key, never a mix). Full-width tiles of grids with
-// nx % 4 == 0 store four cells per lane with 16-byte stores.
-template <class G, int MODE>
-__device__ inline void write_tile(const FillView& v, const typename FillLevel<MODE>::T* sf,
-                                  const typename FillLds<G, MODE>::B* sd, long long x0,
-                                  long long y0) {
-  using L = FillLevel<MODE>;
-  using T = typename L::T;
-  // opaque per call, as in stage_spill: no per-thread addresses hoisted
-  // out of the persistent loop
-  int tid = threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  auto inert_at = [&](int at) {
-    if constexpr (MODE == kModeSpill)
-      return sd[at] == kInertCell;
-    else
-      return sd[at] == L::kInert;
-  };
-  // fill / recon output of one cell
-  auto level_out = [&](int at, size_t g) {
-    float out = unord_bits(sf[at]);
-    if (inert_at(at)) {
-      const float raw = v.src[g];
-      if (raw != raw) out = (MODE == kModeFill) ? raw : -raw;
-    }
-    return (MODE == kModeRecon) ? -out : out;
-  };
-  auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  bool vec = ((v.nx & 3) == 0) && (x0 + G::TX <= v.nx);
-  if constexpr (MODE == kModeSpill)
-    vec = vec && al(v.keys) && al(v.spill);
-  else
-    vec = vec && al(v.state);
-  if (vec) {
-    constexpr int Q = G::TX / 4, NQ = G::TY * Q;
-#pragma unroll 4
-    for (int q = tid; q < NQ; q += G::THREADS) {
-      const int r = q / Q, c = 4 * (q % Q);
-      const long long gy = y0 + r;
-      if (gy >= v.ny) continue;
-      const size_t g = (size_t)gy * v.nx + x0 + c;
-      const int at = G::at(r, c);
-      if constexpr (MODE == kModeSpill) {
-        const T k0 = sf[at], k1 = sf[at + 1], k2 = sf[at + 2], k3 = sf[at + 3];
-        *reinterpret_cast<ulonglong2*>(v.keys + g) = make_ulonglong2(k0, k1);
-        *reinterpret_cast<ulonglong2*>(v.keys + g + 2) = make_ulonglong2(k2, k3);
-        *reinterpret_cast<int4*>(v.spill + g) =
-            make_int4(inert_at(at) ? -1 : (int)(unsigned)k0, inert_at(at + 1) ? -1 : (int)(unsigned)k1,
-                      inert_at(at + 2) ? -1 : (int)(unsigned)k2, inert_at(at + 3) ? -1 : (int)(unsigned)k3);
-      } else {
-        *reinterpret_cast<float4*>(v.state + g) =
-            make_float4(level_out(at, g), level_out(at + 1, g + 1), level_out(at + 2, g + 2),
-                        level_out(at + 3, g + 3));
-      }
-    }
-  } else {
-    for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-      const int r = idx / G::TX, c = idx % G::TX;
-      const long long gx = x0 + c, gy = y0 + r;
-      if (gx >= v.nx || gy >= v.ny) continue;
-      const size_t g = (size_t)gy * v.nx + gx;
-      const int at = G::at(r, c);
-      if constexpr (MODE == kModeSpill) {
-        const T val = sf[at];
-        v.keys[g] = val;
-        v.spill[g] = inert_at(at) ? -1 : (int)(unsigned)val;
-      } else {
-        v.state[g] = level_out(at, g);
-      }
-    }
-  }
-}
-
 // Writes the wavefront's own 64x64 quadrant (tile rows R0.., columns C0..)
 // back, for fill / reconstruction: a quadrant the visit did not change is
 // left as it is in memory. Same values as write_tile.
@@ -948,17 +843,15 @@ __device__ inline void write_quadrant(const FillView& v, const unsigned* sf, con
   }
 }
 
-// The spill pass's version: the wavefront's quadrant of keys and spill
-// indices (one aligned store per key, as in write_tile).
+// The spill pass's version: the wavefront's quadrant of spill indices
+// (the key's index halves; the levels never change), -1 for inert cells.
 template <class G>
 __device__ inline void write_spill_quadrant(const FillView& v, const unsigned long long* sf,
                                             const unsigned char* sd, long long x0, long long y0,
                                             int R0, int C0, int lane) {
-  using T = unsigned long long;
   const long long qx = x0 + C0, qy = y0 + R0;
-  auto spill_of = [&](int at, T k) { return sd[at] == kInertCell ? -1 : (int)(unsigned)k; };
-  if (((v.nx & 3) == 0) && (qx + 64 <= v.nx) && (reinterpret_cast<uintptr_t>(v.keys) & 15) == 0 &&
-      (reinterpret_cast<uintptr_t>(v.spill) & 15) == 0) {
+  auto spill_of = [&](int at) { return sd[at] == kInertCell ? -1 : (int)(unsigned)sf[at]; };
+  if (((v.nx & 3) == 0) && (qx + 64 <= v.nx) && (reinterpret_cast<uintptr_t>(v.spill) & 15) == 0) {
 #pragma unroll 4
     for (int k = 0; k < 16; k++) {
       const int q = lane + 64 * k;
@@ -966,21 +859,14 @@ __device__ inline void write_spill_quadrant(const FillView& v, const unsigned lo
       if (qy + r >= v.ny) continue;
       const size_t g = (size_t)(qy + r) * v.nx + qx + c;
       const int at = G::at(R0 + r, C0 + c);
-      const T k0 = sf[at], k1 = sf[at + 1], k2 = sf[at + 2], k3 = sf[at + 3];
-      *reinterpret_cast<ulonglong2*>(v.keys + g) = make_ulonglong2(k0, k1);
-      *reinterpret_cast<ulonglong2*>(v.keys + g + 2) = make_ulonglong2(k2, k3);
-      *reinterpret_cast<int4*>(v.spill + g) = make_int4(spill_of(at, k0), spill_of(at + 1, k1),
-                                                       spill_of(at + 2, k2), spill_of(at + 3, k3));
+      *reinterpret_cast<int4*>(v.spill + g) =
+          make_int4(spill_of(at), spill_of(at + 1), spill_of(at + 2), spill_of(at + 3));
     }
   } else {
     for (int r = 0; r < 64; r++) {
       const long long gx = qx + lane, gy = qy + r;
       if (gx >= v.nx || gy >= v.ny) continue;
-      const size_t g = (size_t)gy * v.nx + gx;
-      const int at = G::at(R0 + r, C0 + lane);
-      const T val = sf[at];
-      v.keys[g] = val;
-      v.spill[g] = spill_of(at, val);
+      v.spill[(size_t)gy * v.nx + gx] = spill_of(G::at(R0 + r, C0 + lane));
     }
   }
 }
@@ -1005,31 +891,22 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   const unsigned long long c0 = wall_clock64();
   // ---- stage tile + halo
   if constexpr (MODE == kModeSpill) {
-    stage_spill<G>(v, sf, sd, x0, y0, first);
+    stage_spill<G>(v, sf, sd, x0, y0);
   } else {
     stage_levels<G, MODE>(v, sf, sd, x0, y0, first);
   }
   __syncthreads();
   // halo cells whose owner tile's stored state availability differs from
-  // what was staged (recon state is always initialised: the marker)
-  if (MODE != kModeRecon) {
+  // what was staged (the recon marker and the spill indices are always
+  // initialised)
+  if constexpr (MODE == kModeFill) {
     for_stale_halo<G>(first, nvis, [&](int r, int c, bool from_state) {
       const long long gx = x0 + c, gy = y0 + r;
       if (gx < 0 || gx >= v.nx || gy < 0 || gy >= v.ny) return;
       const size_t g = (size_t)gy * v.nx + gx;
       const float raw = v.src[g];
       if (raw != raw) return;  // inert either way
-      if constexpr (MODE == kModeSpill) {
-        const T k = from_state ? v.keys[g]
-                               : ((v.bc[g] == 1) ? KeyLevel::make(raw, (unsigned)g)
-                                                 : KeyLevel::make(v.state[g], 0xffffffffu));
-        sf[G::at(r, c)] = k;
-        // the staged word (and the kind taken from its level) may have been
-        // an unvisited neighbour's uninitialised key
-        sd[G::at(r, c)] = (unsigned)(k >> 32) == ord_bits(raw) ? kSurface : kSubmerged;
-      } else {
-        sf[G::at(r, c)] = ord_bits(from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf));
-      }
+      sf[G::at(r, c)] = ord_bits(from_state ? v.state[g] : ((v.bc[g] == 1) ? raw : kInf));
     });
     __syncthreads();
   }
@@ -1037,35 +914,10 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // anything below lowers it: every lowering from here on is compared with
   // it to decide which neighbours to queue
   save_perimeter<G>(lds.sper, sf);
-  // spill keys from known levels, first visit: an unfilled cell (level ==
-  // its own d) with a strictly lower neighbour keeps (d, own index) at the
-  // fixed point; seeding it there leaves only plateau cells to settle
   // the bound of each cell (kModeSpill); tile cell (0, 0) is grid cell
   // y0 * nx + x0, below 2^31 like every spill index
   const SpillBound bound{reinterpret_cast<const unsigned char*>(lds.sd),
                          (unsigned)(y0 * v.nx + x0), (unsigned)v.nx};
-  if constexpr (MODE == kModeSpill) {
-    if (first) {
-      __syncthreads();  // perimeter saved before any cell is seeded
-      for (int idx = tid; idx < G::TY * G::TX; idx += G::THREADS) {
-        const int r = idx / G::TX, c = idx % G::TX;
-        const T k = sf[G::at(r, c)];
-        // unreached index, at its own level (filled cells lie above d)
-        if ((unsigned)k != 0xffffffffu || sd[G::at(r, c)] != kSurface) continue;
-        const unsigned lev = (unsigned)(k >> 32);
-        unsigned lowest = 0xffffffffu;
-#pragma unroll
-        for (int dr = -1; dr <= 1; dr++)
-#pragma unroll
-          for (int dc = -1; dc <= 1; dc++) {
-            const unsigned l = (unsigned)(sf[G::at(r + dr, c + dc)] >> 32);
-            lowest = l < lowest ? l : lowest;
-          }
-        if (lowest < lev) sf[G::at(r, c)] = bound(G::at(r, c), r, c, k);
-      }
-      __syncthreads();
-    }
-  }
   const unsigned long long c1 = wall_clock64();
   // ---- relax: sweep iterations until the parallel fixed-point test
   // passes (also tried before the first one)
@@ -1128,11 +980,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   // visits skip tiles whose relaxation changed nothing. Inert cells (NaN
   // input) take the input bits.
   if constexpr (MODE == kModeSpill) {
-    // the first visit also seeded cells outside the sweeps: whole tile
-    if (first)
-      write_tile<G, MODE>(v, sf, sd, x0, y0);
-    else if (wave_change)
-      write_spill_quadrant<G>(v, sf, sd, x0, y0, R0, C0, lane);
+    if (wave_change) write_spill_quadrant<G>(v, sf, sd, x0, y0, R0, C0, lane);
   } else {
     // each wavefront writes its own quadrant if its sweeps changed it
     if (first || wave_change) write_quadrant<G, MODE>(v, sf, sd, x0, y0, R0, C0, lane);
@@ -1153,6 +1001,33 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
   if (any) bits |= kSelf;
   if (tid == 0) visit_stats(sync, it, c0, c1, c2);
   return bits;
+}
+
+// ---------------------------------------------------------------- spill relaxation schedule
+// The spill relaxation starts from the labelling's indices (spill_ccl.h),
+// at which only tiles holding a flat cell can break the fixed point: their
+// dirty flag is set. A first visit that finds its tile's flag clear has
+// nothing to do and is skipped; a visit that queues neighbours sets their
+// flags first (spill_mark, before the release fence that precedes the
+// pushes), so a tile whose neighbour changed before its first visit sees
+// the flag set, and one changed later is queued again (a later visit is
+// never skipped). Returns true in every thread when the visit is skipped.
+__device__ inline bool spill_skip(const unsigned* dirty, int t, bool first, unsigned* s_flag) {
+  if (!first) return false;
+  if (threadIdx.x == 0) *s_flag = ld_relaxed(&dirty[t]);
+  __syncthreads();
+  return *s_flag == 0u;
+}
+
+// Thread 0: flags the neighbours a visit queues (bits as run_queue takes them).
+template <class Topo>
+__device__ inline void spill_mark(unsigned* dirty, const Topo& topo, int t, unsigned bits) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 8; k++) {
+    if (!(bits & (1u << k))) continue;
+    const int n = topo.neighbour(t, k);
+    if (n >= 0) __hip_atomic_store(&dirty[n], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 }  // namespace tt

@@ -1,0 +1,534 @@
+// Spill indices of fillsinks from its final levels by connected components
+// (the forward half of the fillsinks adjoint; no reference counterpart).
+//
+// The spill key of a cell is its lexicographic (level, index) minimax over
+// paths from the seeds (fill_visit.h, kModeSpill): the level half is the
+// fill level F, the index half the cell whose DEM value F copies. With F
+// known, most index halves follow from F without any relaxation:
+//
+//  * a surface cell (F == dem, bitwise) that is a bc cell or has a
+//    neighbour at a lower level is a "source": its own spill cell;
+//  * a submerged cell (F > dem) takes the least key of its neighbours, all
+//    of which lie at its level or above, so 8-neighbouring submerged cells
+//    share their level, and every cell of an 8-connected component of
+//    submerged cells has the same spill index: the least index among the
+//    component's equal-level neighbours' keys;
+//  * a surface cell with no lower neighbour ("flat", a plateau at its own
+//    level) is settled only by the relaxation.
+//
+// The labelling below writes the exact index of every source, and for each
+// component the least index of its equal-level source neighbours. That is
+// exact unless a flat cell offers the component a smaller key; every value
+// is an upper bound of the key (an index some path reaches, or none), so
+// the spill relaxation (fill_visit, kModeSpill) reaches the unique fixed
+// point from it, and only tiles holding flat cells can violate it at the
+// start: those are marked dirty and relaxed first.
+//
+// Steps, each a pass over tiles or cells that follows the previous one:
+//   spill_label_tile  classes, union-find of the tile's submerged cells in
+//                     LDS (over row runs), each local root's least source
+//                     candidate; the spill array holds the union-find
+//                     parents (grid indices) of submerged cells, cmin the
+//                     candidates at local roots and a class marker at every
+//                     other cell (kNotRoot, kNotSubmerged)
+//   spill_border_tile unions across the tile's top and left borders, in
+//                     the spill array (lock-free, linking the larger root
+//                     under the smaller)
+//   spill_root_cell   local roots: the path to the root compressed, the
+//                     candidate folded into the root's
+//   spill_link_cell   other submerged cells: parent := the root (their
+//                     parent is a local root, which now points at it)
+//   spill_final_cell  spill := the root's candidate
+// A component spanning many tiles leaves a chain of local roots after the
+// border unions; compressing it once per local root, rather than walking
+// it from every cell, keeps the passes at a few loads per cell.
+#pragma once
+
+#include "fill_visit.h"
+
+namespace tt {
+
+constexpr unsigned kNoIndex = 0xffffffffu;    // no spill index (int -1)
+constexpr unsigned kNotRoot = 0xfffffffeu;    // cmin of a submerged cell that is no local root
+constexpr unsigned kNotSubmerged = 0xfffffffdu;  // cmin of every other cell
+constexpr unsigned kInertLevel = 0xffffffffu;  // order image above every real level
+
+// Cell classes.
+enum : unsigned char {
+  kClsInert = 0,       // NaN DEM or outside the grid
+  kClsSubmerged = 1,   // F > dem
+  kClsSource = 2,      // surface, bc or a lower neighbour
+  kClsFlat = 3         // surface, neither
+};
+
+struct SpillView {
+  const float* F;    // final fill levels
+  const float* dem;
+  const uint8_t* bc;
+  int* spill;        // output; union-find parents of submerged cells before the last step
+  unsigned* cmin;    // per cell: least candidate index (meaningful at roots)
+  unsigned* dirty;   // per relaxation tile: holds a flat cell
+  long long nx, ny;
+  int rtx, rty;      // relaxation tile size in cells
+  int rntx;          // relaxation tiles along x
+};
+
+// ---------------------------------------------------------------- local union-find
+// 16-bit parents, two per 32-bit LDS word; every access is a 32-bit atomic
+// on the word, so concurrent updates of the two halves never lose one.
+__device__ inline unsigned uf_get(unsigned* par, unsigned i) {
+  const unsigned w = __hip_atomic_load(&par[i >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return (i & 1u) ? (w >> 16) : (w & 0xffffu);
+}
+
+// Sets parent(i) := to if parent(i) == expect; returns whether it did.
+__device__ inline bool uf_cas(unsigned* par, unsigned i, unsigned expect, unsigned to) {
+  const int sh = (i & 1u) ? 16 : 0;
+  unsigned old = __hip_atomic_load(&par[i >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  for (;;) {
+    if (((old >> sh) & 0xffffu) != expect) return false;
+    const unsigned nw = (old & ~(0xffffu << sh)) | (to << sh);
+    if (__hip_atomic_compare_exchange_strong(&par[i >> 1], &old, nw, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return true;
+  }
+}
+
+// Root of i with path halving: a node whose parent is not a root gets its
+// grandparent as parent (only if its parent is unchanged meanwhile), so
+// the chains that unions in row order would build stay short.
+__device__ inline unsigned uf_find(unsigned* par, unsigned i) {
+  for (;;) {
+    const unsigned p = uf_get(par, i);
+    if (p == i) return i;
+    const unsigned g = uf_get(par, p);
+    if (g == p) return p;
+    uf_cas(par, i, p, g);
+    i = g;
+  }
+}
+
+__device__ inline void uf_unite(unsigned* par, unsigned a, unsigned b) {
+  for (;;) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      const unsigned t = a;
+      a = b;
+      b = t;
+    }
+    if (uf_cas(par, a, a, b)) return;  // a still a root: linked under b
+  }
+}
+
+// ---------------------------------------------------------------- global union-find
+__device__ inline int gp_load(const int* P, long long x) {
+  return __hip_atomic_load(&P[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving (a non-root only ever gets an ancestor
+// stored; roots change only by the CAS in gp_unite).
+__device__ inline int gp_find(int* P, int x) {
+  for (;;) {
+    const int p = gp_load(P, x);
+    if (p == x) return x;
+    const int g = gp_load(P, p);
+    if (g == p) return p;
+    __hip_atomic_store(&P[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = g;
+  }
+}
+
+// Root of x without writes (once no union runs).
+__device__ inline int gp_root(const int* P, int x) {
+  for (;;) {
+    const int p = gp_load(P, x);
+    if (p == x) return x;
+    x = p;
+  }
+}
+
+__device__ inline void gp_unite(int* P, int a, int b) {
+  for (;;) {
+    a = gp_find(P, a);
+    b = gp_find(P, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    int expect = a;
+    if (__hip_atomic_compare_exchange_strong(&P[a], &expect, b, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+// ---------------------------------------------------------------- tile labelling
+// The tile is processed by rows: a wavefront takes one 64-cell segment of a
+// row at a time (lane = column), so the submerged cells of a segment are a
+// 64-bit ballot and a cell's run start is a bit scan of it.
+template <int TX, int TY>
+struct SpillLabelLds {
+  static constexpr int QX = TX / 64;      // 64-cell segments per row
+  static constexpr int P2 = TX + 4;       // pitch of the tile + 2-cell halo (levels)
+  static constexpr int C2 = (TY + 4) * P2;
+  static constexpr int P1 = TX + 2;       // pitch of the tile + 1-cell halo (classes)
+  static constexpr int C1 = (TY + 2) * P1;
+  static_assert(TX % 64 == 0 && TX * TY <= 65536, "whole segments, 16-bit local parents");
+  union {
+    unsigned lev[C2];        // levels, tile + 2-cell halo (classification)
+    // unions to make, two 16-bit run starts each: at most one per cell (a
+    // cell makes two only as a one-cell run, whose right neighbour makes none)
+    unsigned task[TX * TY];
+    unsigned cand[TX * TY];  // least candidate of each local root (afterwards)
+  };
+  unsigned char cls[C1];
+  unsigned char eq[TX * TY];  // submerged tile cells: bit k = neighbour k is an equal-level source
+  unsigned par[TX * TY / 2];  // 16-bit parents, two per word
+  unsigned short starts[TX * TY / 2];  // the run starts (runs are separated by a cell)
+  unsigned long long sub[TY * QX];  // submerged cells of each segment
+  unsigned flat, ntask, nstart;
+  __device__ static constexpr int l2(int r, int c) { return (r + 2) * P2 + (c + 2); }
+  __device__ static constexpr int l1(int r, int c) { return (r + 1) * P1 + (c + 1); }
+};
+
+// Neighbour k = 0..7 of the eq bits: (dr, dc) in row-major order, centre skipped.
+__device__ constexpr int kEqDr[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+__device__ constexpr int kEqDc[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+// Run starts (a run: consecutive submerged cells of a row) of segment k of
+// row r, and the column of the start of the run through submerged cell
+// (r, c).
+template <int TX, int TY>
+__device__ inline unsigned long long run_starts(const SpillLabelLds<TX, TY>& s, int r, int k) {
+  const unsigned long long m = s.sub[r * (TX / 64) + k];
+  const unsigned long long carry = k > 0 ? (s.sub[r * (TX / 64) + k - 1] >> 63) : 0ull;
+  return m & ~((m << 1) | carry);
+}
+// (r, c) must be submerged, so a start exists at or before c; the scan
+// stops at the row's first segment regardless.
+template <int TX, int TY>
+__device__ inline int run_start(const SpillLabelLds<TX, TY>& s, int r, int c) {
+  int k = c >> 6;
+  unsigned long long st = run_starts(s, r, k) & (~0ull >> (63 - (c & 63)));
+  while (st == 0ull && k > 0) st = run_starts(s, r, --k);  // began in an earlier segment
+  return st ? 64 * k + 63 - __clzll(st) : 0;
+}
+template <int TX, int TY>
+__device__ inline bool sub_at(const SpillLabelLds<TX, TY>& s, int r, int c) {
+  return (s.sub[r * (TX / 64) + (c >> 6)] >> (c & 63)) & 1ull;
+}
+
+// Labels the tile at (x0, y0) (TX x TY cells, NT threads, all threads of the
+// workgroup call it together).
+// GLOBAL: the first of the whole-grid steps (a label tile lies inside a
+// relaxation tile and only sets that tile's dirty flag, cleared
+// beforehand, when it holds a flat cell).
+// Otherwise the tile is a relaxation tile of its own and labelled alone:
+// each component's cells get the component's least candidate as their
+// spill index at once, exact for a component inside the tile, an upper
+// bound for one that continues across the tile's edge; the dirty flag is
+// written either way, set when the tile holds a flat cell or a submerged
+// cell with a submerged neighbour outside the tile. Grid indices fit in
+// 32 bits (nx * ny < 2^31).
+template <int TX, int TY, int NT, bool GLOBAL>
+__device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0l,
+                                 SpillLabelLds<TX, TY>& s) {
+  using S = SpillLabelLds<TX, TY>;
+  constexpr int QX = S::QX, NW = NT / 64;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));  // opaque: no addresses hoisted out of a persistent loop
+  const int lane = tid & 63, wave = tid >> 6;
+  const int x0 = (int)x0l, y0 = (int)y0l, nx = (int)v.nx, ny = (int)v.ny;
+  if (tid == 0) {
+    s.flat = 0u;
+    s.ntask = 0u;
+    s.nstart = 0u;
+  }
+  // levels of the tile + 2-cell halo; first classes of the tile + 1-cell
+  // halo: inert, submerged, or surface (| 8 for a bc cell). A row is QX
+  // segments plus the four halo columns (lanes 0..3 of one more pass);
+  // every load of a batch of rows is issued before any value is used.
+  {
+    constexpr int ROWS = TY + 4, PASSES = QX + 1, U = (ROWS + 2 * NW - 1) / (2 * NW);
+    for (int r0 = wave * U; r0 < ROWS; r0 += NW * U) {
+      float d[U][PASSES], f[U][PASSES];
+      unsigned char b[U][PASSES];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < PASSES; j++) {
+          const int r = r0 + u - 2;
+          const int c = j < QX ? 64 * j + lane : (lane < 2 ? lane - 2 : TX + lane - 2);
+          const int gx = x0 + c, gy = y0 + r;
+          d[u][j] = __builtin_nanf("");
+          f[u][j] = 0.f;
+          b[u][j] = 0;
+          if (r0 + u < ROWS && (j < QX || lane < 4) && gx >= 0 && gy >= 0 && gx < nx &&
+              gy < ny) {
+            const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)gx;
+            d[u][j] = v.dem[g];
+            f[u][j] = v.F[g];
+            b[u][j] = v.bc[g];
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < PASSES; j++) {
+          const int r = r0 + u - 2;
+          const int c = j < QX ? 64 * j + lane : (lane < 2 ? lane - 2 : TX + lane - 2);
+          if (r0 + u >= ROWS || (j == QX && lane >= 4)) continue;
+          unsigned lev = kInertLevel;
+          unsigned char cls = kClsInert;
+          if (d[u][j] == d[u][j]) {  // false outside the grid too
+            lev = ord_bits(f[u][j]);
+            cls = lev != ord_bits(d[u][j]) ? kClsSubmerged
+                                           : (unsigned char)(kClsSource | (b[u][j] == 1 ? 8 : 0));
+          }
+          s.lev[S::l2(r, c)] = lev;
+          if (r >= -1 && r <= TY && c >= -1 && c <= TX) s.cls[S::l1(r, c)] = cls;
+        }
+    }
+  }
+  __syncthreads();
+  // surface cells of the tile + 1-cell halo: source (bc, or a lower
+  // neighbour) or flat
+  bool any_flat = false;
+  for (int w = wave; w < (TY + 2) * (QX + 1); w += NW) {
+    const int r = w / (QX + 1) - 1, j = w % (QX + 1);
+    const int c = j < QX ? 64 * j + lane : (lane == 0 ? -1 : TX);
+    if (j == QX && lane >= 2) continue;
+    const int at = S::l1(r, c);
+    const unsigned char c0 = s.cls[at];
+    if ((c0 & 7) != kClsSource || (c0 & 8)) {
+      if (c0 & 8) s.cls[at] = kClsSource;
+      continue;
+    }
+    const unsigned lev = s.lev[S::l2(r, c)];
+    bool src = false;
+#pragma unroll
+    for (int n = 0; n < 8; n++) src |= s.lev[S::l2(r + kEqDr[n], c + kEqDc[n])] < lev;
+    s.cls[at] = src ? kClsSource : kClsFlat;
+    any_flat |= !src && r >= 0 && r < TY && c >= 0 && c < TX;
+  }
+  if (any_flat) s.flat = 1u;
+  __syncthreads();
+  // equal-level source neighbours of the submerged tile cells, and the
+  // submerged mask of each segment
+  for (int w = wave; w < TY * QX; w += NW) {
+    const int r = w / QX, c = 64 * (w % QX) + lane;
+    const bool sub = s.cls[S::l1(r, c)] == kClsSubmerged;
+    unsigned char m = 0;
+    bool crosses = false;
+    if (sub) {
+      const unsigned lev = s.lev[S::l2(r, c)];
+#pragma unroll
+      for (int n = 0; n < 8; n++) {
+        const int rr = r + kEqDr[n], cc = c + kEqDc[n];
+        const unsigned char nc = s.cls[S::l1(rr, cc)];
+        if (nc == kClsSource && s.lev[S::l2(rr, cc)] == lev) m |= (unsigned char)(1u << n);
+        crosses |= nc == kClsSubmerged && (rr < 0 || cc < 0 || rr >= TY || cc >= TX);
+      }
+    }
+    if (!GLOBAL && crosses) s.flat = 1u;
+    s.eq[r * TX + c] = m;
+    const unsigned long long mask = __ballot(sub);
+    if (lane == 0) s.sub[w] = mask;
+  }
+  __syncthreads();
+  // levels no longer needed. Parents: every submerged cell points at the
+  // start of its run (a root). The run starts and the unions to make are
+  // collected in lists, so that the dependent LDS round trips of the unions
+  // and finds run across the whole workgroup at once rather than one row
+  // after another: unions of the runs of rows r - 1 and r that touch
+  // (8-connected), once per overlap, at its first column, and at a run's
+  // end where the touching run above does not reach over it (else the
+  // overlap covered it). A segment's masks are wavefront-uniform words;
+  // bit `lane` is the cell.
+  for (int w = wave; w < TY * QX; w += NW) {
+    const int r = w / QX, k = w % QX, c = 64 * k + lane;
+    const int i = r * TX + c;
+    const unsigned long long m = s.sub[w];
+    const bool sub = (m >> lane) & 1ull;
+    const unsigned long long ml = (m << 1) | (k > 0 ? s.sub[w - 1] >> 63 : 0ull);
+    const unsigned long long mr = (m >> 1) | (k + 1 < QX ? s.sub[w + 1] << 63 : 0ull);
+    const bool start = sub && !((ml >> lane) & 1ull), end = sub && !((mr >> lane) & 1ull);
+    const unsigned a = sub ? (unsigned)(r * TX + run_start(s, r, c)) : (unsigned)i;
+    const unsigned hi = (unsigned)__shfl_down((int)a, 1);
+    if (!(lane & 1)) s.par[i >> 1] = a | (hi << 16);
+    // run starts
+    const unsigned long long sm = __ballot(start);
+    unsigned base = 0;
+    if (lane == 0 && sm) base = atomicAdd(&s.nstart, (unsigned)__popcll(sm));
+    base = (unsigned)__shfl((int)base, 0);
+    if (start) s.starts[base + __popcll(sm & ((1ull << lane) - 1ull))] = (unsigned short)i;
+    // unions with the row above
+    bool v_link = false, l_link = false, r_link = false;
+    if (r > 0 && sub) {
+      const unsigned long long u = s.sub[w - QX];
+      const unsigned long long ul = (u << 1) | (k > 0 ? s.sub[w - QX - 1] >> 63 : 0ull);
+      const unsigned long long ur = (u >> 1) | (k + 1 < QX ? s.sub[w - QX + 1] << 63 : 0ull);
+      const bool up = (u >> lane) & 1ull, up_l = (ul >> lane) & 1ull, up_r = (ur >> lane) & 1ull;
+      v_link = up && (start || !up_l);
+      l_link = start && up_l && !up;
+      r_link = end && up_r && !up;
+    }
+    const int nl = (int)v_link + (int)l_link + (int)r_link;
+    int pre = nl;
+    // inclusive prefix sum of the lanes' task counts
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(pre, d);
+      if (lane >= d) pre += o;
+    }
+    const int total = __shfl(pre, 63);
+    unsigned tb = 0;
+    if (lane == 0 && total) tb = atomicAdd(&s.ntask, (unsigned)total);
+    tb = (unsigned)__shfl((int)tb, 0) + (unsigned)(pre - nl);
+    if (v_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c)) << 16);
+    if (l_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c - 1)) << 16);
+    if (r_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c + 1)) << 16);
+  }
+  __syncthreads();
+  const int ntask = (int)s.ntask, nstart = (int)s.nstart;
+  for (int t = tid; t < ntask; t += NT) {
+    const unsigned tk = s.task[t];
+    uf_unite(s.par, tk & 0xffffu, tk >> 16);
+  }
+  __syncthreads();
+  // run starts point straight at their root (no union runs any more, so
+  // concurrent updates all store the root); the task words become the
+  // candidates
+  for (int t = tid; t < nstart; t += NT) {
+    const unsigned i = s.starts[t];
+    const unsigned root = uf_find(s.par, i);
+    for (;;) {
+      const unsigned p = uf_get(s.par, i);
+      if (p == root || uf_cas(s.par, i, p, root)) break;
+    }
+  }
+  for (int i = tid; i < TX * TY; i += NT) s.cand[i] = kNoIndex;
+  __syncthreads();
+  // root of submerged cell i: its run start's parent
+  auto root_of = [&](unsigned i) { return uf_get(s.par, uf_get(s.par, i)); };
+  // each submerged cell's least candidate onto its local root
+  for (int i = tid; i < TX * TY; i += NT) {
+    const unsigned m = s.eq[i];
+    if (!m) continue;
+    const int r = i / TX, c = i % TX;
+    unsigned best = kNoIndex;
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+      if (m & (1u << n)) {
+        const unsigned g =
+            (unsigned)(y0 + r + kEqDr[n]) * (unsigned)nx + (unsigned)(x0 + c + kEqDc[n]);
+        best = g < best ? g : best;
+      }
+    atomicMin(&s.cand[root_of((unsigned)i)], best);
+  }
+  __syncthreads();
+  // write back: spill = own index (sources), none (flat, inert), and for a
+  // submerged cell the local root's grid index with cmin = the candidate at
+  // local roots, a class marker elsewhere (GLOBAL), or the component's
+  // candidate
+  for (int w = wave; w < TY * QX; w += NW) {
+    const int r = w / QX, c = 64 * (w % QX) + lane;
+    const int gx = x0 + c, gy = y0 + r;
+    if (gx >= nx || gy >= ny) continue;
+    const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)gx;
+    const unsigned char cls = s.cls[S::l1(r, c)];
+    int sp;
+    unsigned cm;
+    if (cls == kClsSubmerged) {
+      const unsigned i = (unsigned)(r * TX + c);
+      const unsigned root = root_of(i);
+      if (GLOBAL) {
+        sp = (int)((unsigned)(y0 + (int)root / TX) * (unsigned)nx + (unsigned)(x0 + (int)root % TX));
+        cm = root == i ? s.cand[i] : kNotRoot;
+      } else {
+        sp = (int)s.cand[root];
+        cm = 0u;
+      }
+    } else {
+      sp = cls == kClsSource ? (int)g : -1;
+      cm = kNotSubmerged;
+    }
+    v.spill[g] = sp;
+    if (GLOBAL) v.cmin[g] = cm;
+  }
+  if (tid == 0 && (!GLOBAL || s.flat)) {
+    const long long rt = (y0l / v.rty) * v.rntx + x0l / v.rtx;
+    __hip_atomic_store(&v.dirty[rt], s.flat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Unions across the tile's top border (its row 0 against the row above)
+// and left border (its column 0 against the column to the left): every
+// cross-tile pair of 8-neighbours is taken by the lower or the right tile.
+// Along a border, runs of submerged cells are united once per overlap (at
+// its first cell) and at the runs' diagonal ends, as within a tile (runs
+// cut at the tile's corners still cover every pair).
+template <int TX, int TY, int NT>
+__device__ void spill_border_tile(const SpillView& v, long long x0, long long y0) {
+  auto sub = [&](long long x, long long y) {
+    return x >= 0 && y >= 0 && x < v.nx && y < v.ny && v.cmin[(size_t)y * v.nx + x] != kNotSubmerged;
+  };
+  for (int k = threadIdx.x; k < TX + TY; k += NT) {
+    const bool top = k < TX;
+    // own line: the tile's row 0 (along x) or column 0 (along y); the other
+    // line is the row above or the column to the left
+    const int i = top ? k : k - TX, len = top ? TX : TY;
+    const long long gx = top ? x0 + i : x0, gy = top ? y0 : y0 + i;
+    if (gx >= v.nx || gy >= v.ny || (top ? gy == 0 : gx == 0)) continue;
+    if (!sub(gx, gy)) continue;
+    // cells of the own and the other line at offset j along the border
+    auto own = [&](int j) { return top ? sub(gx + j, gy) : sub(gx, gy + j); };
+    auto oth = [&](int j) { return top ? sub(gx + j, gy - 1) : sub(gx - 1, gy + j); };
+    auto link = [&](int j) {
+      const long long qx = top ? gx + j : gx - 1, qy = top ? gy - 1 : gy + j;
+      gp_unite(v.spill, (int)((size_t)gy * v.nx + gx), (int)((size_t)qy * v.nx + qx));
+    };
+    const bool start = i == 0 || !own(-1);
+    const bool end = i == len - 1 || !own(1);
+    if (oth(0) && (start || !oth(-1))) link(0);
+    if (start && oth(-1)) link(-1);
+    if (end && oth(1)) link(1);
+  }
+}
+
+// Local root g: every node on its path gets the root as parent (once no
+// union runs, every store to a node writes the same value, its root, so
+// concurrent compressions agree), and its candidate folds into the root's.
+// A local root that is not a global root receives no candidate, so its own
+// cmin is read unchanged.
+__device__ inline void spill_root_cell(const SpillView& v, size_t g) {
+  const unsigned c = v.cmin[g];
+  if (c == kNotRoot || c == kNotSubmerged) return;
+  const int root = gp_root(v.spill, (int)g);
+  if (root == (int)g) return;
+  for (int x = (int)g; x != root;) {
+    const int p = gp_load(v.spill, x);
+    __hip_atomic_store(&v.spill[x], root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+  }
+  if (c != kNoIndex) atomicMin(&v.cmin[root], c);
+}
+
+// Submerged cell g that is no local root: parent := its local root's
+// parent, the root (local roots are only read in this step).
+__device__ inline void spill_link_cell(const SpillView& v, size_t g) {
+  if (v.cmin[g] != kNotRoot) return;
+  v.spill[g] = v.spill[v.spill[g]];
+}
+
+// Submerged cell g: spill := its root's candidate (its parent is the root;
+// every thread reads only its own cell's parent).
+__device__ inline void spill_final_cell(const SpillView& v, size_t g) {
+  if (v.cmin[g] == kNotSubmerged) return;
+  v.spill[g] = (int)v.cmin[v.spill[g]];
+}
+
+}  // namespace tt
