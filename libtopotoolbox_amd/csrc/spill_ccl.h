@@ -295,49 +295,95 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
     }
   }
   __syncthreads();
+  // The row loops below take U rows (or segments) of a wavefront at a time
+  // and issue all their LDS loads before using any: the rows are
+  // independent, and a wavefront that walked them one by one would wait
+  // out each row's dependent loads.
+  constexpr int U = (TY * QX) / NW >= 32 ? 8 : 4;
   // surface cells of the tile + 1-cell halo: source (bc, or a lower
   // neighbour) or flat
-  bool any_flat = false;
-  for (int w = wave; w < (TY + 2) * (QX + 1); w += NW) {
-    const int r = w / (QX + 1) - 1, j = w % (QX + 1);
-    const int c = j < QX ? 64 * j + lane : (lane == 0 ? -1 : TX);
-    if (j == QX && lane >= 2) continue;
-    const int at = S::l1(r, c);
-    const unsigned char c0 = s.cls[at];
-    if ((c0 & 7) != kClsSource || (c0 & 8)) {
-      if (c0 & 8) s.cls[at] = kClsSource;
-      continue;
-    }
-    const unsigned lev = s.lev[S::l2(r, c)];
-    bool src = false;
+  {
+    constexpr int NB = (TY + 2) * (QX + 1);
+    bool any_flat = false;
+    for (int w0 = wave; w0 < NB; w0 += NW * U) {
+      unsigned char c0[U];
+      unsigned lv[U], nb[U][8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) src |= s.lev[S::l2(r + kEqDr[n], c + kEqDc[n])] < lev;
-    s.cls[at] = src ? kClsSource : kClsFlat;
-    any_flat |= !src && r >= 0 && r < TY && c >= 0 && c < TX;
+      for (int u = 0; u < U; u++) {
+        const int w = w0 + u * NW, r = w / (QX + 1) - 1, j = w % (QX + 1);
+        const int c = j < QX ? 64 * j + lane : (lane == 0 ? -1 : TX);
+        c0[u] = kClsInert;
+        lv[u] = 0u;
+#pragma unroll
+        for (int n = 0; n < 8; n++) nb[u][n] = kInertLevel;
+        if (w < NB && (j < QX || lane < 2)) {
+          c0[u] = s.cls[S::l1(r, c)];
+          lv[u] = s.lev[S::l2(r, c)];
+#pragma unroll
+          for (int n = 0; n < 8; n++) nb[u][n] = s.lev[S::l2(r + kEqDr[n], c + kEqDc[n])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int w = w0 + u * NW, r = w / (QX + 1) - 1, j = w % (QX + 1);
+        const int c = j < QX ? 64 * j + lane : (lane == 0 ? -1 : TX);
+        if ((c0[u] & 7) != kClsSource) continue;  // inert, submerged, or outside this pass
+        bool src = (c0[u] & 8) != 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) src |= nb[u][n] < lv[u];
+        s.cls[S::l1(r, c)] = src ? kClsSource : kClsFlat;
+        any_flat |= !src && r >= 0 && r < TY && c >= 0 && c < TX && w < NB;
+      }
+    }
+    if (any_flat) s.flat = 1u;
   }
-  if (any_flat) s.flat = 1u;
   __syncthreads();
   // equal-level source neighbours of the submerged tile cells, and the
   // submerged mask of each segment
-  for (int w = wave; w < TY * QX; w += NW) {
-    const int r = w / QX, c = 64 * (w % QX) + lane;
-    const bool sub = s.cls[S::l1(r, c)] == kClsSubmerged;
-    unsigned char m = 0;
+  constexpr int NS = TY * QX;  // segments
+  {
     bool crosses = false;
-    if (sub) {
-      const unsigned lev = s.lev[S::l2(r, c)];
+    for (int w0 = wave; w0 < NS; w0 += NW * U) {
+      unsigned char c0[U], nc[U][8];
+      unsigned lv[U], nl[U][8];
 #pragma unroll
-      for (int n = 0; n < 8; n++) {
-        const int rr = r + kEqDr[n], cc = c + kEqDc[n];
-        const unsigned char nc = s.cls[S::l1(rr, cc)];
-        if (nc == kClsSource && s.lev[S::l2(rr, cc)] == lev) m |= (unsigned char)(1u << n);
-        crosses |= nc == kClsSubmerged && (rr < 0 || cc < 0 || rr >= TY || cc >= TX);
+      for (int u = 0; u < U; u++) {
+        const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+        c0[u] = kClsInert;
+        lv[u] = 0u;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          nc[u][n] = kClsInert;
+          nl[u][n] = kInertLevel;
+        }
+        if (w < NS) {
+          c0[u] = s.cls[S::l1(r, c)];
+          lv[u] = s.lev[S::l2(r, c)];
+#pragma unroll
+          for (int n = 0; n < 8; n++) {
+            nc[u][n] = s.cls[S::l1(r + kEqDr[n], c + kEqDc[n])];
+            nl[u][n] = s.lev[S::l2(r + kEqDr[n], c + kEqDc[n])];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+        if (w >= NS) continue;  // uniform over the wavefront
+        const bool sub = c0[u] == kClsSubmerged;
+        unsigned char m = 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          const int rr = r + kEqDr[n], cc = c + kEqDc[n];
+          if (sub && nc[u][n] == kClsSource && nl[u][n] == lv[u]) m |= (unsigned char)(1u << n);
+          crosses |= sub && nc[u][n] == kClsSubmerged && (rr < 0 || cc < 0 || rr >= TY || cc >= TX);
+        }
+        s.eq[r * TX + c] = m;
+        const unsigned long long mask = __ballot(sub);
+        if (lane == 0) s.sub[w] = mask;
       }
     }
     if (!GLOBAL && crosses) s.flat = 1u;
-    s.eq[r * TX + c] = m;
-    const unsigned long long mask = __ballot(sub);
-    if (lane == 0) s.sub[w] = mask;
   }
   __syncthreads();
   // levels no longer needed. Parents: every submerged cell points at the
@@ -348,49 +394,67 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
   // (8-connected), once per overlap, at its first column, and at a run's
   // end where the touching run above does not reach over it (else the
   // overlap covered it). A segment's masks are wavefront-uniform words;
-  // bit `lane` is the cell.
-  for (int w = wave; w < TY * QX; w += NW) {
-    const int r = w / QX, k = w % QX, c = 64 * k + lane;
-    const int i = r * TX + c;
-    const unsigned long long m = s.sub[w];
-    const bool sub = (m >> lane) & 1ull;
-    const unsigned long long ml = (m << 1) | (k > 0 ? s.sub[w - 1] >> 63 : 0ull);
-    const unsigned long long mr = (m >> 1) | (k + 1 < QX ? s.sub[w + 1] << 63 : 0ull);
-    const bool start = sub && !((ml >> lane) & 1ull), end = sub && !((mr >> lane) & 1ull);
-    const unsigned a = sub ? (unsigned)(r * TX + run_start(s, r, c)) : (unsigned)i;
-    const unsigned hi = (unsigned)__shfl_down((int)a, 1);
-    if (!(lane & 1)) s.par[i >> 1] = a | (hi << 16);
-    // run starts
-    const unsigned long long sm = __ballot(start);
-    unsigned base = 0;
-    if (lane == 0 && sm) base = atomicAdd(&s.nstart, (unsigned)__popcll(sm));
-    base = (unsigned)__shfl((int)base, 0);
-    if (start) s.starts[base + __popcll(sm & ((1ull << lane) - 1ull))] = (unsigned short)i;
-    // unions with the row above
-    bool v_link = false, l_link = false, r_link = false;
-    if (r > 0 && sub) {
-      const unsigned long long u = s.sub[w - QX];
-      const unsigned long long ul = (u << 1) | (k > 0 ? s.sub[w - QX - 1] >> 63 : 0ull);
-      const unsigned long long ur = (u >> 1) | (k + 1 < QX ? s.sub[w - QX + 1] << 63 : 0ull);
-      const bool up = (u >> lane) & 1ull, up_l = (ul >> lane) & 1ull, up_r = (ur >> lane) & 1ull;
-      v_link = up && (start || !up_l);
-      l_link = start && up_l && !up;
-      r_link = end && up_r && !up;
+  // bit `lane` is the cell. One LDS atomic per list reserves the slots of
+  // U segments.
+  for (int w0 = wave; w0 < NS; w0 += NW * U) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned a[U];
+    unsigned long long sm[U], b1[U], b2[U];
+    bool vl[U], ll[U], rl[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int w = w0 + u * NW, r = w / QX, k = w % QX, c = 64 * k + lane, i = r * TX + c;
+      sm[u] = b1[u] = b2[u] = 0ull;
+      vl[u] = ll[u] = rl[u] = false;
+      a[u] = (unsigned)i;
+      if (w >= NS) continue;  // uniform over the wavefront
+      const unsigned long long m = s.sub[w];
+      const bool sub = (m >> lane) & 1ull;
+      const unsigned long long ml = (m << 1) | (k > 0 ? s.sub[w - 1] >> 63 : 0ull);
+      const unsigned long long mr = (m >> 1) | (k + 1 < QX ? s.sub[w + 1] << 63 : 0ull);
+      const bool start = sub && !((ml >> lane) & 1ull), end = sub && !((mr >> lane) & 1ull);
+      if (sub) a[u] = (unsigned)(r * TX + run_start(s, r, c));
+      const unsigned hi = (unsigned)__shfl_down((int)a[u], 1);
+      if (!(lane & 1)) s.par[i >> 1] = a[u] | (hi << 16);
+      sm[u] = __ballot(start);
+      if (r > 0 && sub) {
+        const unsigned long long up = s.sub[w - QX];
+        const unsigned long long ul = (up << 1) | (k > 0 ? s.sub[w - QX - 1] >> 63 : 0ull);
+        const unsigned long long ur = (up >> 1) | (k + 1 < QX ? s.sub[w - QX + 1] << 63 : 0ull);
+        const bool u0 = (up >> lane) & 1ull, u_l = (ul >> lane) & 1ull, u_r = (ur >> lane) & 1ull;
+        vl[u] = u0 && (start || !u_l);
+        ll[u] = start && u_l && !u0;
+        rl[u] = end && u_r && !u0;
+      }
+      const int nl = (int)vl[u] + (int)ll[u] + (int)rl[u];
+      b1[u] = __ballot(nl >= 1);
+      b2[u] = __ballot(nl >= 2);
     }
-    const int nl = (int)v_link + (int)l_link + (int)r_link;
-    int pre = nl;
-    // inclusive prefix sum of the lanes' task counts
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(pre, d);
-      if (lane >= d) pre += o;
+    unsigned nst = 0u, ntk = 0u;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      nst += (unsigned)__popcll(sm[u]);
+      ntk += (unsigned)(__popcll(b1[u]) + __popcll(b2[u]));
     }
-    const int total = __shfl(pre, 63);
-    unsigned tb = 0;
-    if (lane == 0 && total) tb = atomicAdd(&s.ntask, (unsigned)total);
-    tb = (unsigned)__shfl((int)tb, 0) + (unsigned)(pre - nl);
-    if (v_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c)) << 16);
-    if (l_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c - 1)) << 16);
-    if (r_link) s.task[tb++] = a | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c + 1)) << 16);
+    unsigned bs = 0u, bt = 0u;
+    if (lane == 0) {
+      if (nst) bs = atomicAdd(&s.nstart, nst);
+      if (ntk) bt = atomicAdd(&s.ntask, ntk);
+    }
+    bs = (unsigned)__shfl((int)bs, 0);
+    bt = (unsigned)__shfl((int)bt, 0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+      if ((sm[u] >> lane) & 1ull)
+        s.starts[bs + (unsigned)__popcll(sm[u] & below)] = (unsigned short)a[u];
+      unsigned t = bt + (unsigned)(__popcll(b1[u] & below) + __popcll(b2[u] & below));
+      if (vl[u]) s.task[t++] = a[u] | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c)) << 16);
+      if (ll[u]) s.task[t++] = a[u] | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c - 1)) << 16);
+      if (rl[u]) s.task[t++] = a[u] | ((unsigned)((r - 1) * TX + run_start(s, r - 1, c + 1)) << 16);
+      bs += (unsigned)__popcll(sm[u]);
+      bt += (unsigned)(__popcll(b1[u]) + __popcll(b2[u]));
+    }
   }
   __syncthreads();
   const int ntask = (int)s.ntask, nstart = (int)s.nstart;
@@ -412,8 +476,12 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
   }
   for (int i = tid; i < TX * TY; i += NT) s.cand[i] = kNoIndex;
   __syncthreads();
-  // root of submerged cell i: its run start's parent
-  auto root_of = [&](unsigned i) { return uf_get(s.par, uf_get(s.par, i)); };
+  // root of submerged cell i: its run start's parent (the parents no
+  // longer change: plain loads)
+  auto par_of = [&](unsigned i) {
+    const unsigned wd = s.par[i >> 1];
+    return (i & 1u) ? (wd >> 16) : (wd & 0xffffu);
+  };
   // each submerged cell's least candidate onto its local root
   for (int i = tid; i < TX * TY; i += NT) {
     const unsigned m = s.eq[i];
@@ -427,37 +495,52 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
             (unsigned)(y0 + r + kEqDr[n]) * (unsigned)nx + (unsigned)(x0 + c + kEqDc[n]);
         best = g < best ? g : best;
       }
-    atomicMin(&s.cand[root_of((unsigned)i)], best);
+    atomicMin(&s.cand[par_of(par_of((unsigned)i))], best);
   }
   __syncthreads();
   // write back: spill = own index (sources), none (flat, inert), and for a
   // submerged cell the local root's grid index with cmin = the candidate at
   // local roots, a class marker elsewhere (GLOBAL), or the component's
   // candidate
-  for (int w = wave; w < TY * QX; w += NW) {
-    const int r = w / QX, c = 64 * (w % QX) + lane;
-    const int gx = x0 + c, gy = y0 + r;
-    if (gx >= nx || gy >= ny) continue;
-    const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)gx;
-    const unsigned char cls = s.cls[S::l1(r, c)];
-    int sp;
-    unsigned cm;
-    if (cls == kClsSubmerged) {
-      const unsigned i = (unsigned)(r * TX + c);
-      const unsigned root = root_of(i);
-      if (GLOBAL) {
-        sp = (int)((unsigned)(y0 + (int)root / TX) * (unsigned)nx + (unsigned)(x0 + (int)root % TX));
-        cm = root == i ? s.cand[i] : kNotRoot;
-      } else {
-        sp = (int)s.cand[root];
-        cm = 0u;
-      }
-    } else {
-      sp = cls == kClsSource ? (int)g : -1;
-      cm = kNotSubmerged;
+  for (int w0 = wave; w0 < NS; w0 += NW * U) {
+    unsigned char cl[U];
+    unsigned root[U], cd[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+      cl[u] = w < NS ? s.cls[S::l1(r, c)] : (unsigned char)kClsInert;
+      root[u] = cl[u] == kClsSubmerged ? par_of((unsigned)(r * TX + c)) : 0u;
     }
-    v.spill[g] = sp;
-    if (GLOBAL) v.cmin[g] = cm;
+#pragma unroll
+    for (int u = 0; u < U; u++) root[u] = cl[u] == kClsSubmerged ? par_of(root[u]) : 0u;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+      cd[u] = cl[u] == kClsSubmerged ? s.cand[GLOBAL ? (unsigned)(r * TX + c) : root[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int w = w0 + u * NW, r = w / QX, c = 64 * (w % QX) + lane;
+      const int gx = x0 + c, gy = y0 + r;
+      if (w >= NS || gx >= nx || gy >= ny) continue;
+      const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)gx;
+      const unsigned i = (unsigned)(r * TX + c);
+      int sp;
+      unsigned cm = kNotSubmerged;
+      if (cl[u] == kClsSubmerged) {
+        if (GLOBAL) {
+          sp = (int)((unsigned)(y0 + (int)root[u] / TX) * (unsigned)nx +
+                     (unsigned)(x0 + (int)root[u] % TX));
+          cm = root[u] == i ? cd[u] : kNotRoot;
+        } else {
+          sp = (int)cd[u];
+        }
+      } else {
+        sp = cl[u] == kClsSource ? (int)g : -1;
+      }
+      v.spill[g] = sp;
+      if (GLOBAL) v.cmin[g] = cm;
+    }
   }
   if (tid == 0 && (!GLOBAL || s.flat)) {
     const long long rt = (y0l / v.rty) * v.rntx + x0l / v.rtx;
