@@ -305,3 +305,85 @@ def test_adjoint_4096(cuda, oracle_lib):
     print(f"fsm FD: {checked} cells checked, {skipped} skipped (classification differs), "
           f"{kinks} derivatives taken at eps 1e-4 (branch switch within 1e-3)")
     assert checked >= 250
+
+
+def spill_stress_cases():
+    """Grids whose submerged components and flat plateaus cross many tiles:
+    a fractal quantised to 5 m steps (large plateaus at their own level:
+    flat cells, settled by the spill relaxation) and the depression
+    generator at 1500 x 1700 (flat-bottomed bowls spanning tiles, a NaN blob
+    under the reference caller convention)."""
+    from libtopotoolbox_amd import synthetic as S
+
+    z = S.fractal(700, 900, 2.4, 71, 300.0)
+    z = (np.round(z / 5.0) * 5.0).astype(np.float32) + np.float32(0.0)
+    out = [("quantised 700x900", z, edge_bc(z.shape))]
+    d = S.dem_depressions(1500, 1700, 72)
+    d, bc, _ = S.nan_to_bc(d, S.edge_bc(1500, 1700))
+    out.append(("depressions 1500x1700", d, bc))
+    return out
+
+
+@pytest.mark.parametrize("tile", [(2, 2), (2, 1), (1, 1)])
+def test_spill_labelling_stress(cuda, oracle_lib, tile):
+    """Spill index bit-exact to the oracle's lexicographic Priority-Flood on
+    grids where components and plateaus cross label and relaxation tiles."""
+    import libtopotoolbox_amd as tt
+    from libtopotoolbox_amd import ops
+
+    tt.set_tile_shape(*tile)
+    try:
+        for name, dem, bc in spill_stress_cases():
+            want, wspill = oracle_lib.fillsinks_spill(dem, bc)
+            (out, spill), st = ops.fillsinks_spill(to_gpu(dem), to_gpu(bc), stats=True)
+            assert st["error"] == 0
+            np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32),
+                                          err_msg=name)
+            np.testing.assert_array_equal(spill.cpu().numpy().astype(np.int64), wspill,
+                                          err_msg=f"{name} tile {tile}")
+    finally:
+        tt.set_tile_shape(2, 2)
+
+
+def test_spill_labelling_stress_batch(cuda, oracle_lib):
+    """The same grids as items of one batch (tile-by-tile labelling, the
+    relaxation settling components that cross tile edges)."""
+    from libtopotoolbox_amd.batched import Batch
+
+    cases = spill_stress_cases()
+    shapes = [d.shape for _, d, _ in cases]
+    rng = np.random.default_rng(73)
+    gs = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    b = Batch("fillsinks", shapes, tile=(1, 2))
+    b.load([d for _, d, _ in cases], bc=[bc for _, _, bc in cases], g_out=gs)
+    assert b.run(stats=True)["error"] == 0
+    r = b.outputs()
+    for k, (name, dem, bc) in enumerate(cases):
+        want, wspill = oracle_lib.fillsinks_spill(dem, bc)
+        np.testing.assert_array_equal(r["output"][k].cpu().numpy().view(np.uint32),
+                                      want.view(np.uint32), err_msg=name)
+        np.testing.assert_array_equal(r["spill"][k].cpu().numpy().astype(np.int64), wspill,
+                                      err_msg=name)
+        w = oracle_lib.fillsinks_adjoint_f64(wspill, gs[k].astype(np.float64))
+        assert_scatter_close(r["g_dem"][k].cpu().numpy(), w, gs[k], name)
+
+
+@pytest.mark.slow
+def test_spill_8192_depressions(cuda, oracle_lib):
+    """Spill index at 8192^2 on the fill workload's DEM (about 1.2M submerged
+    components, 170K flat cells, lakes across many tiles): bit-exact to the
+    oracle's lexicographic Priority-Flood."""
+    from libtopotoolbox_amd import ops, synthetic as S
+
+    n = 8192
+    dem = S.dem_depressions(n, n, 2)
+    dem, bc, _ = S.nan_to_bc(dem, S.edge_bc(n, n))
+    (out, spill), st = ops.fillsinks_spill(to_gpu(dem), to_gpu(bc), stats=True)
+    assert st["error"] == 0
+    out = out.cpu().numpy()
+    spill = spill.cpu().numpy()
+    want, wspill = oracle_lib.fillsinks_spill(dem, bc)
+    assert_bitwise(out, want, "8192^2 levels")
+    bad = np.flatnonzero(spill.reshape(-1).astype(np.int64) != wspill.reshape(-1))
+    assert bad.size == 0, f"{bad.size} spill indices differ, first at {bad[:5]}"
+    print(f"8192^2 spill index bit-exact; relaxation stats {st}")
