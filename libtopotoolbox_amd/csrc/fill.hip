@@ -47,25 +47,23 @@ __global__ __launch_bounds__(kLabelThreads) void spill_border_kernel(SpillView v
       v, (long long)blockIdx.x * kLabelTile, (long long)blockIdx.y * kLabelTile);
 }
 
-__global__ void spill_root_kernel(SpillView v) {
-  const size_t n = (size_t)v.nx * v.ny;
-  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-       g += (size_t)gridDim.x * blockDim.x)
-    spill_root_cell(v, g);
-}
-
-__global__ void spill_link_kernel(SpillView v) {
-  const size_t n = (size_t)v.nx * v.ny;
-  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-       g += (size_t)gridDim.x * blockDim.x)
-    spill_link_cell(v, g);
-}
-
-__global__ void spill_final_kernel(SpillView v) {
-  const size_t n = (size_t)v.nx * v.ny;
-  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-       g += (size_t)gridDim.x * blockDim.x)
-    spill_final_cell(v, g);
+// The per-cell steps: each thread takes four cells' cmin words at once
+// (one 16-byte load; cmin is 256-byte aligned), most of which say "nothing
+// to do here".
+template <void (*CELL)(const SpillView&, size_t, unsigned)>
+__global__ void spill_cells_kernel(SpillView v) {
+  const size_t n = (size_t)v.nx * v.ny, n4 = n / 4;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const uint4* c4 = reinterpret_cast<const uint4*>(v.cmin);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
+    const uint4 c = c4[q];
+    CELL(v, 4 * q, c.x);
+    CELL(v, 4 * q + 1, c.y);
+    CELL(v, 4 * q + 2, c.z);
+    CELL(v, 4 * q + 3, c.w);
+  }
+  for (size_t g = 4 * n4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride)
+    CELL(v, g, v.cmin[g]);
 }
 
 template <class G, int MODE>
@@ -160,15 +158,15 @@ hipError_t run_fill_spill(float* out, int* spill, const float* dem, const uint8_
   sv.rntx = rntx;
   const dim3 tiles((unsigned)((nx + kLabelTile - 1) / kLabelTile),
                    (unsigned)((ny + kLabelTile - 1) / kLabelTile));
-  size_t cb = (n + 255) / 256;
-  const int cells_grid = (int)(cb > 8192 ? 8192 : cb);
+  size_t cb = (n / 4 + 255) / 256;
+  const int cells_grid = (int)(cb > 8192 ? 8192 : (cb < 1 ? 1 : cb));
   zero_words<<<(int)((ntiles + 255) / 256 > 1024 ? 1024 : (ntiles + 255) / 256), 256, 0, stream>>>(
       sv.dirty, ntiles);
   spill_label_kernel<<<tiles, kLabelThreads, 0, stream>>>(sv);
   spill_border_kernel<<<tiles, kLabelThreads, 0, stream>>>(sv);
-  spill_root_kernel<<<cells_grid, 256, 0, stream>>>(sv);
-  spill_link_kernel<<<cells_grid, 256, 0, stream>>>(sv);
-  spill_final_kernel<<<cells_grid, 256, 0, stream>>>(sv);
+  spill_cells_kernel<spill_root_cell><<<cells_grid, 256, 0, stream>>>(sv);
+  spill_cells_kernel<spill_final_cell><<<cells_grid, 256, 0, stream>>>(sv);
+  spill_cells_kernel<spill_final_root><<<cells_grid, 256, 0, stream>>>(sv);
   e = hipGetLastError();
   if (e == hipSuccess) {
     // relaxation of the indices the labelling left as upper bounds

@@ -36,9 +36,10 @@
 //                     under the smaller)
 //   spill_root_cell   local roots: the path to the root compressed, the
 //                     candidate folded into the root's
-//   spill_link_cell   other submerged cells: parent := the root (their
-//                     parent is a local root, which now points at it)
-//   spill_final_cell  spill := the root's candidate
+//   spill_final_cell  other submerged cells: spill := the root's candidate
+//                     (their parent is a local root, which now points at
+//                     the root)
+//   spill_final_root  local roots: spill := the root's candidate
 // A component spanning many tiles leaves a chain of local roots after the
 // border unions; compressing it once per local root, rather than walking
 // it from every cell, keeps the passes at a few loads per cell.
@@ -249,10 +250,67 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
     s.nstart = 0u;
   }
   // levels of the tile + 2-cell halo; first classes of the tile + 1-cell
-  // halo: inert, submerged, or surface (| 8 for a bc cell). A row is QX
-  // segments plus the four halo columns (lanes 0..3 of one more pass);
-  // every load of a batch of rows is issued before any value is used.
-  {
+  // halo: inert, submerged, or surface (| 8 for a bc cell). Every load of a
+  // batch is issued before any value is used.
+  auto put = [&](int r, int c, float d, float f, unsigned char b) {
+    unsigned lev = kInertLevel;
+    unsigned char cls = kClsInert;
+    if (d == d) {  // false outside the grid too
+      lev = ord_bits(f);
+      cls = lev != ord_bits(d) ? kClsSubmerged : (unsigned char)(kClsSource | (b == 1 ? 8 : 0));
+    }
+    s.lev[S::l2(r, c)] = lev;
+    if (r >= -1 && r <= TY && c >= -1 && c <= TX) s.cls[S::l1(r, c)] = cls;
+  };
+  auto al = [](const void* p, uintptr_t m) { return (reinterpret_cast<uintptr_t>(p) & m) == 0; };
+  if ((nx & 3) == 0 && x0 + TX <= nx && al(v.dem, 15) && al(v.F, 15) && al(v.bc, 3)) {
+    // the tile's columns four cells per lane (16-byte loads), rows -2..TY+1
+    constexpr int Q = TX / 4, NV = (TY + 4) * Q, UV = (NV + 2 * NT - 1) / (2 * NT);
+    for (int base = 0; base < NV; base += NT * UV) {
+      float4 dv[UV], fv[UV];
+      uchar4 bv[UV];
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int k = base + u * NT + tid, gy = y0 + k / Q - 2;
+        const float nan = __builtin_nanf("");
+        dv[u] = make_float4(nan, nan, nan, nan);
+        fv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        bv[u] = make_uchar4(0, 0, 0, 0);
+        if (k < NV && gy >= 0 && gy < ny) {
+          const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)(x0 + 4 * (k % Q));
+          dv[u] = *reinterpret_cast<const float4*>(v.dem + g);
+          fv[u] = *reinterpret_cast<const float4*>(v.F + g);
+          bv[u] = *reinterpret_cast<const uchar4*>(v.bc + g);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UV; u++) {
+        const int k = base + u * NT + tid;
+        if (k >= NV) continue;
+        const int r = k / Q - 2, c = 4 * (k % Q);
+        put(r, c, dv[u].x, fv[u].x, bv[u].x);
+        put(r, c + 1, dv[u].y, fv[u].y, bv[u].y);
+        put(r, c + 2, dv[u].z, fv[u].z, bv[u].z);
+        put(r, c + 3, dv[u].w, fv[u].w, bv[u].w);
+      }
+    }
+    // the halo columns -2, -1, TX, TX + 1
+    for (int k = tid; k < 4 * (TY + 4); k += NT) {
+      const int r = k / 4 - 2, j = k % 4, c = j < 2 ? j - 2 : TX + j - 2;
+      const int gx = x0 + c, gy = y0 + r;
+      float d = __builtin_nanf(""), f = 0.f;
+      unsigned char b = 0;
+      if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
+        const unsigned g = (unsigned)gy * (unsigned)nx + (unsigned)gx;
+        d = v.dem[g];
+        f = v.F[g];
+        b = v.bc[g];
+      }
+      put(r, c, d, f, b);
+    }
+  } else {
+    // a row is QX segments plus the four halo columns (lanes 0..3 of one
+    // more pass)
     constexpr int ROWS = TY + 4, PASSES = QX + 1, U = (ROWS + 2 * NW - 1) / (2 * NW);
     for (int r0 = wave * U; r0 < ROWS; r0 += NW * U) {
       float d[U][PASSES], f[U][PASSES];
@@ -282,15 +340,7 @@ __device__ void spill_label_tile(const SpillView& v, long long x0l, long long y0
           const int r = r0 + u - 2;
           const int c = j < QX ? 64 * j + lane : (lane < 2 ? lane - 2 : TX + lane - 2);
           if (r0 + u >= ROWS || (j == QX && lane >= 4)) continue;
-          unsigned lev = kInertLevel;
-          unsigned char cls = kClsInert;
-          if (d[u][j] == d[u][j]) {  // false outside the grid too
-            lev = ord_bits(f[u][j]);
-            cls = lev != ord_bits(d[u][j]) ? kClsSubmerged
-                                           : (unsigned char)(kClsSource | (b[u][j] == 1 ? 8 : 0));
-          }
-          s.lev[S::l2(r, c)] = lev;
-          if (r >= -1 && r <= TY && c >= -1 && c <= TX) s.cls[S::l1(r, c)] = cls;
+          put(r, c, d[u][j], f[u][j], b[u][j]);
         }
     }
   }
@@ -587,8 +637,7 @@ __device__ void spill_border_tile(const SpillView& v, long long x0, long long y0
 // concurrent compressions agree), and its candidate folds into the root's.
 // A local root that is not a global root receives no candidate, so its own
 // cmin is read unchanged.
-__device__ inline void spill_root_cell(const SpillView& v, size_t g) {
-  const unsigned c = v.cmin[g];
+__device__ inline void spill_root_cell(const SpillView& v, size_t g, unsigned c) {
   if (c == kNotRoot || c == kNotSubmerged) return;
   const int root = gp_root(v.spill, (int)g);
   if (root == (int)g) return;
@@ -600,17 +649,18 @@ __device__ inline void spill_root_cell(const SpillView& v, size_t g) {
   if (c != kNoIndex) atomicMin(&v.cmin[root], c);
 }
 
-// Submerged cell g that is no local root: parent := its local root's
-// parent, the root (local roots are only read in this step).
-__device__ inline void spill_link_cell(const SpillView& v, size_t g) {
-  if (v.cmin[g] != kNotRoot) return;
-  v.spill[g] = v.spill[v.spill[g]];
+// Submerged cell g that is no local root: spill := the candidate of its
+// root, its local root's parent (local roots are only read in this step,
+// and a cell's own word only by its own thread).
+__device__ inline void spill_final_cell(const SpillView& v, size_t g, unsigned c) {
+  if (c != kNotRoot) return;
+  v.spill[g] = (int)v.cmin[v.spill[v.spill[g]]];
 }
 
-// Submerged cell g: spill := its root's candidate (its parent is the root;
-// every thread reads only its own cell's parent).
-__device__ inline void spill_final_cell(const SpillView& v, size_t g) {
-  if (v.cmin[g] == kNotSubmerged) return;
+// Local root g (the last step): spill := its root's candidate (its parent
+// is the root, or itself).
+__device__ inline void spill_final_root(const SpillView& v, size_t g, unsigned c) {
+  if (c == kNotRoot || c == kNotSubmerged) return;
   v.spill[g] = (int)v.cmin[v.spill[g]];
 }
 
