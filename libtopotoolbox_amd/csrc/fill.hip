@@ -2,8 +2,12 @@
 // work-queue kernel over fill_visit (fill_visit.h states the semantics),
 // and the spill indices of the fillsinks adjoint (labelling of spill_ccl.h,
 // then the spill relaxation from the dirty tiles).
+#include <stdlib.h>
+#include <string.h>
+
 #include <mutex>
 
+#include "prio_queue.h"
 #include "spill_ccl.h"
 #include "workspace.h"
 
@@ -76,6 +80,146 @@ static hipError_t launch_fill(FillArgs a, hipStream_t stream, RunStats* stats,
                            G::TX, G::TY, a.ntx, a.nty, scratch, max_visits);
 }
 
+// ---------------------------------------------------------------- lowest level first
+// kModeFill on one grid with the priority queue of prio_queue.h.
+struct FillPrioArgs {
+  FillView v;
+  int ntx, nty;
+  PrioQueue q;
+  unsigned max_local;
+};
+
+template <class G>
+__global__ __launch_bounds__(G::THREADS) void fill_prio_persistent(FillPrioArgs a) {
+  __shared__ FillLds<G, kModeFill> lds;
+  const GridTopo topo{a.ntx, a.nty};
+  run_prio_queue(a.q, topo, [&](int t, bool first, unsigned nvis, unsigned* lev) -> unsigned {
+    const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
+    return fill_visit<G, kModeFill>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds,
+                                    lev);
+  });
+}
+
+__device__ inline unsigned wave_min(unsigned x) {
+  for (int o = 32; o > 0; o >>= 1) x = min(x, (unsigned)__shfl_xor((int)x, o));
+  return x;
+}
+__device__ inline unsigned wave_max(unsigned x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, (unsigned)__shfl_xor((int)x, o));
+  return x;
+}
+
+// One workgroup per tile: the key of the tile's lowest seed (boundary cell
+// with a value) into q.seed, and the finite DEM range into the control
+// block (pad[0] = ~key(min), pad[1] = key(max), raised from zero).
+constexpr int kSeedThreads = 256;
+template <class G>
+__global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int ntx, PrioQueue q) {
+  const int t = blockIdx.x;
+  const long long x0 = (long long)(t % ntx) * G::TX, y0 = (long long)(t / ntx) * G::TY;
+  unsigned smin = ~0u, fmin = ~0u, fmax = 0u;
+  for (int i = threadIdx.x; i < G::TX * G::TY; i += kSeedThreads) {
+    const long long gx = x0 + i % G::TX, gy = y0 + i / G::TX;
+    if (gx >= v.nx || gy >= v.ny) continue;
+    const size_t g = (size_t)gy * v.nx + gx;
+    const float d = v.src[g];
+    if (d != d) continue;
+    const unsigned k = ord_bits(d);
+    if (v.bc[g] == 1) smin = min(smin, k);
+    if (d - d == 0.f) {  // finite
+      fmin = min(fmin, k);
+      fmax = max(fmax, k);
+    }
+  }
+  __shared__ unsigned red[3][kSeedThreads / 64];
+  smin = wave_min(smin);
+  fmin = wave_min(fmin);
+  fmax = wave_max(fmax);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wave] = smin;
+    red[1][wave] = fmin;
+    red[2][wave] = fmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kSeedThreads / 64; w++) {
+      smin = min(smin, red[0][w]);
+      fmin = min(fmin, red[1][w]);
+      fmax = max(fmax, red[2][w]);
+    }
+    q.seed[t] = smin;
+    if (fmin != ~0u) atomicMax(&q.sync->pad[0], ~fmin);
+    if (fmax != 0u) atomicMax(&q.sync->pad[1], fmax);
+  }
+}
+
+// Initial queue: every tile, at the bucket of its lowest seed, or in the
+// unreached bucket when it holds none.
+__global__ void prio_queue_kernel(PrioQueue q) {
+  const PrioScale sc = prio_scale(q.sync);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    q.sync->pending = (unsigned)q.ntiles;
+    q.sync->tail = (unsigned)q.ntiles;
+  }
+  for (int t = i; t < q.ntiles; t += gridDim.x * blockDim.x) {
+    const unsigned k = q.seed[t];
+    const int b = k == ~0u ? kPrioUnreached : sc.bucket(k);
+    q.state[t] = kQueued | ((unsigned)b << 8);
+    q.visited[t] = 0u;
+    pq_mark(q, t, b);
+  }
+}
+
+static int launch_blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+template <class G>
+static hipError_t launch_fill_prio(const FillView& v, hipStream_t stream, RunStats* stats) {
+  FillPrioArgs a{};
+  a.v = v;
+  a.ntx = (int)((v.nx + G::TX - 1) / G::TX);
+  a.nty = (int)((v.ny + G::TY - 1) / G::TY);
+  a.max_local = 1u << 16;
+  const int ntiles = a.ntx * a.nty;
+  int grid = 0;
+  hipError_t e = persistent_grid((const void*)fill_prio_persistent<G>, G::THREADS, ntiles, &grid);
+  if (e != hipSuccess) return e;
+  const size_t words = prio_workspace_words(ntiles);
+  void* mem = nullptr;
+  e = hipMallocAsync(&mem, words * sizeof(unsigned), stream);
+  if (e != hipSuccess) return e;
+  // finish() reads the control block's statistics and frees the memory
+  Workspace ws;
+  ws.base = mem;
+  ws.owned = true;
+  ws.stream = stream;
+  a.q = prio_layout(mem, ntiles);
+  ws.q.sync = a.q.sync;
+  ws.q.ntiles = ntiles;
+  a.q.wait_limit = wait_limit_ticks();
+  a.q.stats = stats ? 1 : 0;
+  a.q.window = grid;
+  e = sticky_error_word(&a.q.sticky);
+  if (e == hipSuccess) {
+    zero_words<<<launch_blocks(words), 256, 0, stream>>>(static_cast<unsigned*>(mem), words);
+    prio_seed_kernel<G><<<ntiles, kSeedThreads, 0, stream>>>(v, a.ntx, a.q);
+    prio_queue_kernel<<<launch_blocks((size_t)ntiles), 256, 0, stream>>>(a.q);
+    fill_prio_persistent<G><<<grid, G::THREADS, 0, stream>>>(a);
+    e = hipGetLastError();
+  }
+  if (stats) {
+    stats->grid = grid;
+    stats->tile_x = G::TX;
+    stats->tile_y = G::TY;
+  }
+  hipError_t f = ws.finish(stats);
+  return e != hipSuccess ? e : f;
+}
+
 template <int MODE>
 static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, RunStats* stats) {
   if constexpr (MODE == kModeSpill) {
@@ -93,6 +237,39 @@ static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, Ru
   }
 }
 
+// Schedule of sink filling on one grid: the lowest-level-first queue pays
+// off once tiles outnumber the workgroups by kPrioMinTilesPerWorker or
+// more (fewer tiles: every tile has a workgroup of its own most of the
+// time, and the FIFO queue's cheaper pops win). TT_FILL_SCHEDULE=fifo or
+// =priority overrides the choice (both reach the same result).
+constexpr int kPrioMinTilesPerWorker = 4;
+
+enum FillSchedule { kSchedAuto, kSchedFifo, kSchedPriority };
+
+static FillSchedule fill_schedule() {
+  const char* e = getenv("TT_FILL_SCHEDULE");
+  if (e && strcmp(e, "fifo") == 0) return kSchedFifo;
+  if (e && strcmp(e, "priority") == 0) return kSchedPriority;
+  return kSchedAuto;
+}
+
+template <class G>
+static hipError_t launch_fill_scheduled(FillArgs a, hipStream_t stream, RunStats* stats) {
+  const FillSchedule s = fill_schedule();
+  if (s == kSchedAuto) {
+    const long long ntiles = ((a.v.nx + G::TX - 1) / G::TX) * ((a.v.ny + G::TY - 1) / G::TY);
+    int grid = 0;
+    hipError_t e =
+        persistent_grid((const void*)fill_prio_persistent<G>, G::THREADS, 1 << 30, &grid);
+    if (e != hipSuccess) return e;
+    if (ntiles < (long long)kPrioMinTilesPerWorker * grid)
+      return launch_fill<G, kModeFill>(a, stream, stats);
+    return launch_fill_prio<G>(a.v, stream, stats);
+  }
+  if (s == kSchedFifo) return launch_fill<G, kModeFill>(a, stream, stats);
+  return launch_fill_prio<G>(a.v, stream, stats);
+}
+
 hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
                     int mode, TileShape ts, hipStream_t stream, RunStats* stats) {
   if (nx <= 0 || ny <= 0) return hipSuccess;
@@ -102,7 +279,12 @@ hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t n
   a.v.bc = bc;
   a.v.nx = nx;
   a.v.ny = ny;
-  if (mode == kModeFill) return dispatch_fill<kModeFill>(a, ts, stream, stats);
+  if (mode == kModeFill) {
+    if (ts.qx == 2 && ts.qy == 2) return launch_fill_scheduled<TileGeom<2, 2>>(a, stream, stats);
+    if (ts.qx == 2 && ts.qy == 1) return launch_fill_scheduled<TileGeom<2, 1>>(a, stream, stats);
+    if (ts.qx == 1 && ts.qy == 2) return launch_fill_scheduled<TileGeom<1, 2>>(a, stream, stats);
+    return launch_fill_scheduled<TileGeom<1, 1>>(a, stream, stats);
+  }
   return dispatch_fill<kModeRecon>(a, ts, stream, stats);
 }
 

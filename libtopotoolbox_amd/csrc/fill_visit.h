@@ -874,11 +874,13 @@ __device__ inline void write_spill_quadrant(const FillView& v, const unsigned lo
 // Relaxes the tile at (x0, y0) of grid v to its local fixed point (at most
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
-// of the workgroup; `sync` receives the visit statistics.
+// of the workgroup; `sync` receives the visit statistics. With `lev`
+// (kModeFill; LDS, 8 words), lev[k] is lowered to the order key of the
+// lowest level the tile pushes onto neighbour k.
 template <class G, int MODE>
 __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long y0, bool first,
                                       unsigned nvis, unsigned max_local, Sync* sync,
-                                      FillLds<G, MODE>& lds) {
+                                      FillLds<G, MODE>& lds, unsigned* lev = nullptr) {
   using L = FillLevel<MODE>;
   using T = typename L::T;
   T* sf = lds.sf;
@@ -994,8 +996,15 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
                                                                          q % G::P - 1, sf[q]);
                                                             else
                                                               dq = sd[q];
-                                                            return L::less(L::hi(dq, sf[p]),
-                                                                           sf[q]);
+                                                            const T to = L::hi(dq, sf[p]);
+                                                            const bool lowers = L::less(to, sf[q]);
+                                                            if constexpr (MODE == kModeFill)
+                                                              if (lowers && lev)
+                                                                atomicMin(&lev[halo_owner<G>(
+                                                                              q / G::P - 1,
+                                                                              q % G::P - 1)],
+                                                                          to);
+                                                            return lowers;
                                                           }),
                            &lds.bits);
   if (any) bits |= kSelf;

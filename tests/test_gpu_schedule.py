@@ -1,0 +1,104 @@
+"""GPU parity of both schedules of the single-grid sink filling: the
+lowest-level-first queue and the FIFO queue each reach the oracle's fixed
+point bit for bit, on every tile shape, whatever the automatic choice
+would have been for the grid size (TT_FILL_SCHEDULE forces one)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import assert_bitwise, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(2, 2), (2, 1), (1, 2), (1, 1)]
+
+
+@pytest.fixture(params=["priority", "fifo"])
+def schedule(request):
+    old = os.environ.get("TT_FILL_SCHEDULE")
+    os.environ["TT_FILL_SCHEDULE"] = request.param
+    yield request.param
+    if old is None:
+        os.environ.pop("TT_FILL_SCHEDULE", None)
+    else:
+        os.environ["TT_FILL_SCHEDULE"] = old
+
+
+def edge_bc(shape):
+    bc = np.zeros(shape, np.uint8)
+    bc[0, :] = bc[-1, :] = 1
+    bc[:, 0] = bc[:, -1] = 1
+    return bc
+
+
+def fill(dem, bc, tile):
+    import libtopotoolbox_amd as tt
+
+    tt.set_tile_shape(*tile)
+    d = torch.from_numpy(np.ascontiguousarray(dem)).cuda()
+    b = torch.from_numpy(np.ascontiguousarray(bc)).cuda()
+    out, st = tt.fillsinks(d, b, stats=True)
+    assert st["error"] == 0, st
+    return out.cpu().numpy(), st
+
+
+@pytest.mark.parametrize("name", ["fillsinks_random_s0", "fillsinks_nan_raw",
+                                  "fillsinks_nan_convention", "fillsinks_flats"])
+def test_golden(cuda, schedule, name):
+    g = load_golden(name)
+    for tile in TILES:
+        out, _ = fill(g["dem"], g["bc"], tile)
+        assert_bitwise(out, g["out"], f"{name} {schedule} tile {tile}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (37, 1), (65, 129), (257, 300), (700, 900)])
+def test_shapes(cuda, oracle_lib, schedule, shape):
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    dem = oracle_lib.grid_random(shape[0] * 1000 + shape[1], shape)
+    dem[rng.uniform(size=shape) < 0.02] = np.nan
+    bc = edge_bc(shape)
+    want = oracle_lib.fillsinks_pf(dem, bc)
+    for tile in TILES:
+        out, _ = fill(dem, bc, tile)
+        assert_bitwise(out, want, f"shape {shape} {schedule} tile {tile}")
+
+
+def test_degenerate_levels(cuda, oracle_lib, schedule):
+    """Level ranges the bucket map must survive: one level everywhere (an
+    empty range), huge finite magnitudes (a range that overflows), +-INF and
+    NaN cells, no seed at all, seeds only in the middle, every cell a seed."""
+    rng = np.random.default_rng(11)
+    shape = (300, 410)
+    cases = {
+        "constant": np.full(shape, 5.0, np.float32),
+        "huge": rng.choice(np.float32([-3e38, 3e38, 0.0, 1e30]), size=shape).astype(np.float32),
+        "specials": np.where(rng.uniform(size=shape) < 0.1,
+                             rng.choice(np.float32([np.inf, -np.inf, np.nan]), size=shape),
+                             rng.uniform(0, 50, size=shape)).astype(np.float32),
+    }
+    middle = np.zeros(shape, np.uint8)
+    middle[150, 200] = middle[20:25, 300] = 1
+    bcs = {"edge": edge_bc(shape), "none": np.zeros(shape, np.uint8), "middle": middle,
+           "all": np.ones(shape, np.uint8)}
+    for dname, dem in cases.items():
+        for bname, bc in bcs.items():
+            want, _, conv = oracle_lib.fillsinks(dem, bc)
+            assert conv
+            for tile in ((2, 2), (1, 1)):
+                out, _ = fill(dem, bc, tile)
+                assert_bitwise(out, want, f"{dname} / bc {bname} {schedule} tile {tile}")
+
+
+def test_depressions_1500(cuda, oracle_lib, schedule):
+    """Many tiles per workgroup's worth of queue traffic: 1500 x 1700 with
+    depressions, flat bottoms and a NaN blob (NaN drains to the edge)."""
+    from libtopotoolbox_amd import synthetic as S
+
+    dem = S.dem_depressions(1500, 1700, 72)
+    dem, bc, _ = S.nan_to_bc(dem, S.edge_bc(1500, 1700))
+    want = oracle_lib.fillsinks_pf(dem, bc)
+    for tile in TILES:
+        out, st = fill(dem, bc, tile)
+        assert_bitwise(out, want, f"depressions 1500x1700 {schedule} tile {tile}")
