@@ -457,8 +457,11 @@ def batched_cpu_baseline(op: str, dems, aux, workers: int):
                       f"tiles, one tile per task on {workers} threads", **host_info()}
 
 
-TRAFFIC_KERNELS = {"fill": "fill_persistent", "fsm": "fsm_persistent",
-                   "copy": "copyBuffer"}
+# kernels of one call of each workload (name fragments): the persistent
+# relaxation and, for the fill, the pre-pass that seeds its queue
+TRAFFIC_KERNELS = {"fill": ("fill_prio_persistent", "fill_persistent", "prio_seed_kernel",
+                            "prio_queue_kernel", "zero_words"),
+                   "fsm": ("fsm_persistent",), "copy": ("copyBuffer",)}
 
 
 def _pmc_pass(counter: str, tag: str, bench_args: list) -> dict:
@@ -497,17 +500,22 @@ def measure_traffic(n: int):
     from libtopotoolbox_amd.build import source_hash
 
     def pick(vals, key):
-        rows = [v for k, v in vals.items() if TRAFFIC_KERNELS[key] in k]
-        if not rows:
+        # mean KiB per dispatch of each kernel of the call, by kernel
+        got = {}
+        for k, v in vals.items():
+            for frag in TRAFFIC_KERNELS[key]:
+                if frag in k:
+                    got.setdefault(frag, []).extend(v)
+                    break
+        if not got:
             raise RuntimeError(f"no {TRAFFIC_KERNELS[key]} dispatch in the counter output")
-        flat = [x for r in rows for x in r]
-        return flat
+        return {frag: float(np.mean(v)) for frag, v in got.items()}
 
     copy_kib = 256 * 1024
     cal = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         vals = _pmc_pass(counter, "copy", ["--copy-calibration", "256"])
-        cal[counter] = float(np.mean(pick(vals, "copy")))
+        cal[counter] = pick(vals, "copy")["copyBuffer"]
     scale = {c: copy_kib / v for c, v in cal.items()}
     result = {"calibration": {"copy_kib": copy_kib, "fetch_kib": round(cal["FETCH_SIZE"], 1),
                               "write_kib": round(cal["WRITE_SIZE"], 1),
@@ -523,18 +531,19 @@ def measure_traffic(n: int):
         args = ["--workload", wl, "--size", str(n), "--steps", "3", "--warmup", "0", "--no-cpu",
                 "--no-extra", "--no-adjoint", "--no-batched", "--no-stencils"]
         raw = {c: pick(_pmc_pass(c, wl, args), wl) for c in ("FETCH_SIZE", "WRITE_SIZE")}
-        rd = float(np.mean(raw["FETCH_SIZE"])) * 1024 * scale["FETCH_SIZE"]
-        wr = float(np.mean(raw["WRITE_SIZE"])) * 1024 * scale["WRITE_SIZE"]
+        rd = sum(raw["FETCH_SIZE"].values()) * 1024 * scale["FETCH_SIZE"]
+        wr = sum(raw["WRITE_SIZE"].values()) * 1024 * scale["WRITE_SIZE"]
         algo = BYTES_PER_CELL[wl] * n * n
-        result[wl] = {"kernel": TRAFFIC_KERNELS[wl],
-                      "fetch_kib_raw": [round(x, 1) for x in raw["FETCH_SIZE"]],
-                      "write_kib_raw": [round(x, 1) for x in raw["WRITE_SIZE"]],
+        result[wl] = {"kernels": sorted(raw["FETCH_SIZE"]),
+                      "fetch_kib_raw": {k: round(x, 1) for k, x in raw["FETCH_SIZE"].items()},
+                      "write_kib_raw": {k: round(x, 1) for k, x in raw["WRITE_SIZE"].items()},
                       "read_bytes": int(rd), "write_bytes": int(wr),
                       "bytes_per_launch": int(rd + wr), "algorithmic_bytes": algo,
                       "traffic_over_algorithmic": round((rd + wr) / algo, 2)}
     with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
         json.dump(result, f, indent=1)
-    lines = [f"# HBM traffic per launch: fill_persistent and fsm_persistent, {n}x{n}", "",
+    lines = [f"# HBM traffic per call: fillsinks and excesstopography_fsm2d kernels, {n}x{n}",
+             "",
              "Measured by `python bench.py --traffic` on one MI355X (kernel sources "
              f"{result['source_hash']}, commit {result['commit']}). It runs each counter in a "
              "rocprofv3 pass of its own over a child process:", "", "```"]
@@ -547,13 +556,20 @@ def measure_traffic(n: int):
               f"A plain 256 MiB device copy reads FETCH_SIZE = {cal['FETCH_SIZE']:.1f} KiB and "
               f"WRITE_SIZE = {cal['WRITE_SIZE']:.1f} KiB per dispatch against {copy_kib} KiB moved "
               f"each way: scale factors {scale['FETCH_SIZE']:.4f} and {scale['WRITE_SIZE']:.4f}.",
-              "", "## Per launch (mean over the dispatches, scaled)", "",
-              "| kernel | FETCH_SIZE raw KiB | WRITE_SIZE raw KiB | HBM read | HBM write | total | "
-              "algorithmic | ratio |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+              "", "## Per call (mean KiB per dispatch of each kernel, summed over the kernels "
+              "of one call, scaled)", "",
+              "| workload | kernel | FETCH_SIZE raw KiB | WRITE_SIZE raw KiB |",
+              "|---|---|---:|---:|"]
     for wl in ("fill", "fsm"):
         r = result[wl]
-        lines.append(f"| `{r['kernel']}` ({wl}) | {np.mean(r['fetch_kib_raw']):.0f} | "
-                     f"{np.mean(r['write_kib_raw']):.0f} | {r['read_bytes'] / 1e9:.2f} GB | "
+        for k in r["kernels"]:
+            lines.append(f"| {wl} | `{k}` | {r['fetch_kib_raw'][k]:.0f} | "
+                         f"{r['write_kib_raw'].get(k, 0.0):.0f} |")
+    lines += ["", "| workload | HBM read | HBM write | total | algorithmic | ratio |",
+              "|---|---:|---:|---:|---:|---:|"]
+    for wl in ("fill", "fsm"):
+        r = result[wl]
+        lines.append(f"| {wl} | {r['read_bytes'] / 1e9:.2f} GB | "
                      f"{r['write_bytes'] / 1e9:.2f} GB | {r['bytes_per_launch'] / 1e9:.2f} GB | "
                      f"{r['algorithmic_bytes'] / 1e9:.3f} GB ({BYTES_PER_CELL[wl]} B/cell) | "
                      f"{r['traffic_over_algorithmic']}x |")

@@ -359,6 +359,19 @@ int tt_batch_run_dev(const void *plan_host, const void *plan_dev,
    topography 2x1 (the fastest measured at 8192^2). */
 int tt_set_tile_shape(int qx, int qy);
 
+/* Work-queue schedule of single-grid sink filling (tt_fillsinks_dev,
+   fillsinks, fillsinks_hybrid, the spill forward's levels) for subsequent
+   calls on this thread; returns nonzero for an unknown value. AUTO (the
+   default) visits queued tiles lowest pending level first when tiles
+   outnumber the persistent workgroups at least four to one and there are
+   at most 512 workgroups, and in FIFO order otherwise; FIFO and PRIORITY
+   force one queue. Every schedule gives the same, bit-exact result. No
+   counterpart in the reference (a GPU scheduling choice). */
+#define TT_FILL_SCHEDULE_AUTO 0
+#define TT_FILL_SCHEDULE_FIFO 1
+#define TT_FILL_SCHEDULE_PRIORITY 2
+int tt_set_fill_schedule(int schedule);
+
 /* Device errors of asynchronous calls. A call made without `stats` returns
    before its kernels end, so a device error (a bounded queue wait that
    expired: bit 1) cannot be in its return code. Every persistent launch

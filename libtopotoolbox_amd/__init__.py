@@ -35,6 +35,7 @@ from .ops import (
     reconstruct_host,
     reconstruct_hybrid_host,
     set_tile_shape,
+    set_fill_schedule,
 )
 
 __all__ = [
@@ -64,6 +65,7 @@ __all__ = [
     "reconstruct_host",
     "reconstruct_hybrid_host",
     "set_tile_shape",
+    "set_fill_schedule",
 ]
 
 

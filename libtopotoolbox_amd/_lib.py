@@ -129,6 +129,7 @@ SIGNATURES = {
     "tt_batch_run_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_size_t, ctypes.c_uint,
                                         ctypes.c_int, _vp, ctypes.POINTER(Stats)]),
     "tt_set_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "tt_set_fill_schedule": (ctypes.c_int, [ctypes.c_int]),
     "tt_device_error": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]),
     "tt_set_wait_limit": (ctypes.c_int, [ctypes.c_uint64]),
     "tt_host_workspace_release": (None, []),

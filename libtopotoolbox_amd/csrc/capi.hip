@@ -153,6 +153,16 @@ int tt_set_tile_shape(int qx, int qy) {
   return 0;
 }
 
+int tt_set_fill_schedule(int schedule) {
+  if (schedule != TT_FILL_SCHEDULE_AUTO && schedule != TT_FILL_SCHEDULE_FIFO &&
+      schedule != TT_FILL_SCHEDULE_PRIORITY) {
+    tt::set_error("tt_set_fill_schedule: unknown schedule %d", schedule);
+    return 1;
+  }
+  tt::set_fill_schedule(static_cast<tt::FillSchedule>(schedule));
+  return 0;
+}
+
 int tt_fillsinks_dev(float* output, const float* dem, const uint8_t* bc, ptrdiff_t nx,
                      ptrdiff_t ny, void* stream, tt_stats* stats) {
   tt::set_error("%s", "");

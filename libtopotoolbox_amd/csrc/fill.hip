@@ -2,9 +2,6 @@
 // work-queue kernel over fill_visit (fill_visit.h states the semantics),
 // and the spill indices of the fillsinks adjoint (labelling of spill_ccl.h,
 // then the spill relaxation from the dirty tiles).
-#include <stdlib.h>
-#include <string.h>
-
 #include <mutex>
 
 #include "prio_queue.h"
@@ -111,24 +108,46 @@ __device__ inline unsigned wave_max(unsigned x) {
 
 // One workgroup per tile: the key of the tile's lowest seed (boundary cell
 // with a value) into q.seed, and the finite DEM range into the control
-// block (pad[0] = ~key(min), pad[1] = key(max), raised from zero).
+// block (pad[0] = ~key(min), pad[1] = key(max), raised from zero). Each
+// thread reads four adjacent cells per row (16-byte DEM and 4-byte bc
+// loads where the row allows).
 constexpr int kSeedThreads = 256;
 template <class G>
 __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int ntx, PrioQueue q) {
+  constexpr int kPerRow = G::TX / 4;              // threads per tile row
+  constexpr int kRows = kSeedThreads / kPerRow;   // rows per pass
+  static_assert(G::TY % kRows == 0, "whole passes");
   const int t = blockIdx.x;
   const long long x0 = (long long)(t % ntx) * G::TX, y0 = (long long)(t / ntx) * G::TY;
+  const long long gx = x0 + 4 * (threadIdx.x % kPerRow);
+  const bool vec = v.nx % 4 == 0 && gx + 3 < v.nx &&
+                   (reinterpret_cast<uintptr_t>(v.src) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(v.bc) & 3) == 0;
   unsigned smin = ~0u, fmin = ~0u, fmax = 0u;
-  for (int i = threadIdx.x; i < G::TX * G::TY; i += kSeedThreads) {
-    const long long gx = x0 + i % G::TX, gy = y0 + i / G::TX;
-    if (gx >= v.nx || gy >= v.ny) continue;
-    const size_t g = (size_t)gy * v.nx + gx;
-    const float d = v.src[g];
-    if (d != d) continue;
+  auto cell = [&](float d, unsigned char b) {
+    if (d != d) return;
     const unsigned k = ord_bits(d);
-    if (v.bc[g] == 1) smin = min(smin, k);
+    if (b == 1) smin = min(smin, k);
     if (d - d == 0.f) {  // finite
       fmin = min(fmin, k);
       fmax = max(fmax, k);
+    }
+  };
+#pragma unroll 4
+  for (int r = threadIdx.x / kPerRow; r < G::TY; r += kRows) {
+    const long long gy = y0 + r;
+    if (gy >= v.ny) break;
+    const size_t g = (size_t)gy * v.nx + gx;
+    if (vec) {
+      const float4 d = *reinterpret_cast<const float4*>(v.src + g);
+      const uchar4 b = *reinterpret_cast<const uchar4*>(v.bc + g);
+      cell(d.x, b.x);
+      cell(d.y, b.y);
+      cell(d.z, b.z);
+      cell(d.w, b.w);
+    } else {
+      for (int k = 0; k < 4; k++)
+        if (gx + k < v.nx) cell(v.src[g + k], v.bc[g + k]);
     }
   }
   __shared__ unsigned red[3][kSeedThreads / 64];
@@ -149,26 +168,41 @@ __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int
       fmax = max(fmax, red[2][w]);
     }
     q.seed[t] = smin;
-    if (fmin != ~0u) atomicMax(&q.sync->pad[0], ~fmin);
-    if (fmax != 0u) atomicMax(&q.sync->pad[1], fmax);
+    // thousands of workgroups raise the same two words: only those that
+    // would raise them issue the atomic
+    if (fmin != ~0u && ld_relaxed(&q.sync->pad[0]) < ~fmin) atomicMax(&q.sync->pad[0], ~fmin);
+    if (fmax != 0u && ld_relaxed(&q.sync->pad[1]) < fmax) atomicMax(&q.sync->pad[1], fmax);
   }
 }
 
 // Initial queue: every tile, at the bucket of its lowest seed, or in the
-// unreached bucket when it holds none.
+// unreached bucket when it holds none. A wavefront takes 64 consecutive
+// tiles (one bitmap word): its unreached tiles, usually most of them, are
+// marked with one word update and one count update.
 __global__ void prio_queue_kernel(PrioQueue q) {
   const PrioScale sc = prio_scale(q.sync);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     q.sync->pending = (unsigned)q.ntiles;
     q.sync->tail = (unsigned)q.ntiles;
   }
-  for (int t = i; t < q.ntiles; t += gridDim.x * blockDim.x) {
-    const unsigned k = q.seed[t];
-    const int b = k == ~0u ? kPrioUnreached : sc.bucket(k);
-    q.state[t] = kQueued | ((unsigned)b << 8);
-    q.visited[t] = 0u;
-    pq_mark(q, t, b);
+  const int stride = gridDim.x * blockDim.x;
+  for (int t0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63; t0 < q.ntiles; t0 += stride) {
+    const int t = t0 + lane;
+    const bool in = t < q.ntiles;
+    int b = kPrioUnreached;
+    if (in) {
+      const unsigned k = q.seed[t];
+      if (k != ~0u) b = sc.bucket(k);
+      q.state[t] = kQueued | ((unsigned)b << 8);
+      q.visited[t] = 0u;
+    }
+    const unsigned long long un = __ballot(in && b == kPrioUnreached);
+    if (in && b != kPrioUnreached) pq_mark(q, t, b);
+    if (lane == 0 && un) {
+      atomicOr(&q.bits[(size_t)kPrioUnreached * q.nwords + (t0 >> 6)], un);
+      atomicAdd(&q.count[kPrioUnreached], __popcll(un));
+    }
   }
 }
 
@@ -240,18 +274,18 @@ static hipError_t dispatch_fill(FillArgs a, TileShape ts, hipStream_t stream, Ru
 // Schedule of sink filling on one grid: the lowest-level-first queue pays
 // off once tiles outnumber the workgroups by kPrioMinTilesPerWorker or
 // more (fewer tiles: every tile has a workgroup of its own most of the
-// time, and the FIFO queue's cheaper pops win). TT_FILL_SCHEDULE=fifo or
-// =priority overrides the choice (both reach the same result).
+// time, and the FIFO queue's cheaper pops win), with at most
+// kPrioMaxWorkers workgroups (more poppers contend for the lowest tiles:
+// 1024 workgroups of 64 x 64 tiles lose to FIFO on 8192 x 8192 grids).
+// set_fill_schedule overrides the choice for the calling thread.
 constexpr int kPrioMinTilesPerWorker = 4;
+constexpr int kPrioMaxWorkers = 512;
 
-enum FillSchedule { kSchedAuto, kSchedFifo, kSchedPriority };
+static thread_local FillSchedule g_fill_schedule = kSchedAuto;
 
-static FillSchedule fill_schedule() {
-  const char* e = getenv("TT_FILL_SCHEDULE");
-  if (e && strcmp(e, "fifo") == 0) return kSchedFifo;
-  if (e && strcmp(e, "priority") == 0) return kSchedPriority;
-  return kSchedAuto;
-}
+void set_fill_schedule(FillSchedule s) { g_fill_schedule = s; }
+
+static FillSchedule fill_schedule() { return g_fill_schedule; }
 
 template <class G>
 static hipError_t launch_fill_scheduled(FillArgs a, hipStream_t stream, RunStats* stats) {
@@ -262,7 +296,7 @@ static hipError_t launch_fill_scheduled(FillArgs a, hipStream_t stream, RunStats
     hipError_t e =
         persistent_grid((const void*)fill_prio_persistent<G>, G::THREADS, 1 << 30, &grid);
     if (e != hipSuccess) return e;
-    if (ntiles < (long long)kPrioMinTilesPerWorker * grid)
+    if (ntiles < (long long)kPrioMinTilesPerWorker * grid || grid > kPrioMaxWorkers)
       return launch_fill<G, kModeFill>(a, stream, stats);
     return launch_fill_prio<G>(a.v, stream, stats);
   }

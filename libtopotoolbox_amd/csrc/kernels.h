@@ -36,6 +36,13 @@ enum FillMode : int {
 hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
                     int mode, TileShape ts, hipStream_t stream, RunStats* stats);
 
+// Work-queue schedule of sink filling on one grid (kModeFill) for later
+// calls on this thread: automatic (lowest level first on large grids, FIFO
+// otherwise), FIFO, or lowest level first. Both queues reach the same
+// result.
+enum FillSchedule : int { kSchedAuto = 0, kSchedFifo = 1, kSchedPriority = 2 };
+void set_fill_schedule(FillSchedule s);
+
 // run_fill (128 x 128 tiles) that stops after about max_visits (>= 1) tile
 // visits: every tile is visited at least once, and the state is a valid
 // partial result (between the fixed point and the initial marker), so a

@@ -21,10 +21,10 @@
 // of its bucket; a moved tile leaves its old bit behind, which a popper
 // discards when the tile's state no longer names that bucket. A pop ranks
 // the set bits by bucket (from the counts), picks one of the lowest
-// `window` of them at random (window = the number of workgroups: with
-// strictly lowest-first pops, hundreds of idle workgroups race for the
-// same few tiles), clears it and claims the tile if its state still says
-// "queued in this bucket". The counts may lag the bitmaps for a moment;
+// `window` of them at random, lower ranks more likely (window = the number
+// of workgroups: with strictly lowest-first pops, hundreds of idle
+// workgroups race for the same few tiles), clears it and claims the tile
+// if its state still says "queued in this bucket". The counts may lag the bitmaps for a moment;
 // they agree again whenever no push or pop is in flight.
 //
 // Termination and ordering follow tile_engine.h: a pending counter of
@@ -185,7 +185,9 @@ __device__ inline int pq_pop(const PrioQueue& q, unsigned rnd, int* bucket) {
     if (total > 0) {
       // rank r among the lowest `window` counted bits -> its bucket
       const int window = total < q.window ? total : q.window;
-      const int r = (int)((rnd >> 8) % (unsigned)window);
+      // the lower of two uniform ranks: lower ranks are more likely
+      const unsigned r2 = (rnd * 747796405u + 2891336453u) >> 8;
+      const int r = min((int)((rnd >> 8) % (unsigned)window), (int)(r2 % (unsigned)window));
       const int src = __ffsll((long long)__ballot(incl > r)) - 1;
       if (lane == src) {
         int acc = incl - sum;

@@ -61,6 +61,17 @@ def set_tile_shape(qx: int, qy: int) -> None:
     _lib.check(_lib.lib().tt_set_tile_shape(qx, qy), "tt_set_tile_shape")
 
 
+FILL_SCHEDULES = {"auto": 0, "fifo": 1, "priority": 2}
+
+
+def set_fill_schedule(schedule: str) -> None:
+    """Work-queue schedule of single-grid fillsinks for later calls on this
+    thread: "auto" (default), "fifo" or "priority" (same results)."""
+    if schedule not in FILL_SCHEDULES:
+        raise ValueError(f"unknown fill schedule {schedule!r}; one of {sorted(FILL_SCHEDULES)}")
+    _lib.check(_lib.lib().tt_set_fill_schedule(FILL_SCHEDULES[schedule]), "tt_set_fill_schedule")
+
+
 def fillsinks(dem: torch.Tensor, bc: torch.Tensor, out: torch.Tensor | None = None,
               stats: bool = False):
     """Filled DEM. ``bc == 1`` marks fixed cells (reference fillsinks)."""

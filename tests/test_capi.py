@@ -4,6 +4,8 @@ import glob
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -105,3 +107,20 @@ def test_capped_entries_check_arguments_without_gpu():
         assert f(None, None, *extra, 64, 64, 10, None, None, None) == 1
         assert "converged" in _lib.last_error()
         assert conv.value == 7
+
+
+def test_fill_schedule_setter_without_gpu():
+    """The schedule setter is host-only: known values are accepted, an
+    unknown one is refused with a message, and the Python wrapper checks
+    names before calling it."""
+    import libtopotoolbox_amd as tt
+    from libtopotoolbox_amd import _lib
+
+    L = _lib.lib()
+    for v in (1, 2, 0):
+        assert L.tt_set_fill_schedule(v) == 0
+    assert L.tt_set_fill_schedule(7) == 1
+    assert "unknown schedule 7" in _lib.last_error()
+    with pytest.raises(ValueError):
+        tt.set_fill_schedule("lifo")
+    tt.set_fill_schedule("auto")

@@ -1,9 +1,7 @@
 """GPU parity of both schedules of the single-grid sink filling: the
 lowest-level-first queue and the FIFO queue each reach the oracle's fixed
 point bit for bit, on every tile shape, whatever the automatic choice
-would have been for the grid size (TT_FILL_SCHEDULE forces one)."""
-import os
-
+would have been for the grid size (tt_set_fill_schedule forces one)."""
 import numpy as np
 import pytest
 import torch
@@ -17,13 +15,11 @@ TILES = [(2, 2), (2, 1), (1, 2), (1, 1)]
 
 @pytest.fixture(params=["priority", "fifo"])
 def schedule(request):
-    old = os.environ.get("TT_FILL_SCHEDULE")
-    os.environ["TT_FILL_SCHEDULE"] = request.param
+    import libtopotoolbox_amd as tt
+
+    tt.set_fill_schedule(request.param)
     yield request.param
-    if old is None:
-        os.environ.pop("TT_FILL_SCHEDULE", None)
-    else:
-        os.environ["TT_FILL_SCHEDULE"] = old
+    tt.set_fill_schedule("auto")
 
 
 def edge_bc(shape):
