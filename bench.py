@@ -540,13 +540,29 @@ def measure_traffic(n: int):
                       "read_bytes": int(rd), "write_bytes": int(wr),
                       "bytes_per_launch": int(rd + wr), "algorithmic_bytes": algo,
                       "traffic_over_algorithmic": round((rd + wr) / algo, 2)}
+    print(json.dumps(result), flush=True)
+    write_traffic_report(result)
+
+
+def write_traffic_report(result: dict):
+    """profiles/traffic.json and profiles/hbm_traffic_8192.md from one
+    measure_traffic result (also from its printed JSON line, with
+    `--traffic-report FILE`, where the GPU run's files do not come back)."""
+    n = result["size"]
+    cal = {"FETCH_SIZE": result["calibration"]["fetch_kib"],
+           "WRITE_SIZE": result["calibration"]["write_kib"]}
+    scale = {"FETCH_SIZE": result["calibration"]["fetch_scale"],
+             "WRITE_SIZE": result["calibration"]["write_scale"]}
+    copy_kib = result["calibration"]["copy_kib"]
     with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
         json.dump(result, f, indent=1)
     lines = [f"# HBM traffic per call: fillsinks and excesstopography_fsm2d kernels, {n}x{n}",
              "",
              "Measured by `python bench.py --traffic` on one MI355X (kernel sources "
-             f"{result['source_hash']}, commit {result['commit']}). It runs each counter in a "
-             "rocprofv3 pass of its own over a child process:", "", "```"]
+             f"{result['source_hash']}"
+             + (f", commit {result['commit']}" if result.get("commit") else "") +
+             "). It runs each counter in a rocprofv3 pass of its own over a child process:",
+             "", "```"]
     lines += ["rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d gpurun_out/pmc_copy_<C> "
               "-o p -- python bench.py --copy-calibration 256",
               "rocprofv3 --pmc {FETCH_SIZE|WRITE_SIZE} --output-format csv -d gpurun_out/pmc_<wl>_<C> "
@@ -577,7 +593,6 @@ def measure_traffic(n: int):
               "library's source stamp equals the one above."]
     with open(os.path.join(ROOT, "profiles", "hbm_traffic_8192.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
-    print(json.dumps(result), flush=True)
 
 
 def main():
@@ -600,12 +615,22 @@ def main():
                     help="measure HBM bytes per launch of the fill and excess-topography "
                          "kernels with rocprofv3 counter passes (child processes) and write "
                          "profiles/traffic.json and profiles/hbm_traffic_8192.md")
+    ap.add_argument("--traffic-report", metavar="FILE",
+                    help="rewrite profiles/traffic.json and profiles/hbm_traffic_8192.md from "
+                         "the JSON line a `--traffic` run printed (no GPU)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.traffic:
         measure_traffic(args.size)
+        return
+    if args.traffic_report:
+        with open(args.traffic_report) as f:
+            rows = [ln for ln in f if ln.startswith("{") and '"calibration"' in ln]
+        if not rows:
+            sys.exit(f"no measure_traffic JSON line in {args.traffic_report}")
+        write_traffic_report(json.loads(rows[-1]))
         return
     if args.copy_calibration:
         import torch
