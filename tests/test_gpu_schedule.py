@@ -98,3 +98,19 @@ def test_depressions_1500(cuda, oracle_lib, schedule):
     for tile in TILES:
         out, st = fill(dem, bc, tile)
         assert_bitwise(out, want, f"depressions 1500x1700 {schedule} tile {tile}")
+
+
+def test_more_tiles_than_one_bitmap_stretch(cuda, oracle_lib, schedule):
+    """4160 x 4170 cells in 64 x 64 tiles: 4290 tiles, more than the 4096
+    one stretch of 64 bitmap words holds, so pops rank and pick across
+    several stretches of a bucket (the 16384^2 fill takes that path with
+    128 x 128 tiles)."""
+    from libtopotoolbox_amd import synthetic as S
+
+    dem = S.dem_depressions(4160, 4170, 31)
+    dem, bc, _ = S.nan_to_bc(dem, S.edge_bc(4160, 4170))
+    want = oracle_lib.fillsinks_pf(dem, bc)
+    out, st = fill(dem, bc, (1, 1))
+    assert st["tile_x"] == 64 and st["tile_y"] == 64
+    assert_bitwise(out, want, f"4160x4170 {schedule} tile (1, 1)")
+
