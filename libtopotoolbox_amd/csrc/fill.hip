@@ -236,7 +236,9 @@ static hipError_t launch_fill_prio(const FillView& v, hipStream_t stream, RunSta
   ws.q.ntiles = ntiles;
   a.q.wait_limit = wait_limit_ticks();
   a.q.stats = stats ? 1 : 0;
-  a.q.window = grid;
+  // measured at 8192^2: 2x the workgroups 1-2% faster than 1x, 1/4x and
+  // 4x slower
+  a.q.window = 2 * grid;
   e = sticky_error_word(&a.q.sticky);
   if (e == hipSuccess) {
     zero_words<<<launch_blocks(words), 256, 0, stream>>>(static_cast<unsigned*>(mem), words);

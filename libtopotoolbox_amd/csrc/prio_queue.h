@@ -21,8 +21,8 @@
 // of its bucket; a moved tile leaves its old bit behind, which a popper
 // discards when the tile's state no longer names that bucket. A pop ranks
 // the set bits by bucket (from the counts), picks one of the lowest
-// `window` of them at random, lower ranks more likely (window = the number
-// of workgroups: with strictly lowest-first pops, hundreds of idle
+// `window` of them at random, lower ranks more likely (window = twice the
+// number of workgroups: with strictly lowest-first pops, hundreds of idle
 // workgroups race for the same few tiles), clears it and claims the tile
 // if its state still says "queued in this bucket". The counts may lag the bitmaps for a moment;
 // they agree again whenever no push or pop is in flight.
