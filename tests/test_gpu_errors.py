@@ -28,7 +28,10 @@ def _pits(oracle_lib, shape=(1024, 1024), seed=11):
     return dem, bc
 
 
-def test_expired_wait_reaches_sticky_word(cuda, oracle_lib):
+@pytest.mark.parametrize("schedule", ["fifo", "priority"])
+def test_expired_wait_reaches_sticky_word(cuda, oracle_lib, schedule):
+    """Both work queues: the FIFO pop's slot wait and the lowest-level-first
+    pop's wait for a queued tile give up at the bound."""
     import libtopotoolbox_amd as tt
     from libtopotoolbox_amd import _lib
 
@@ -37,6 +40,14 @@ def test_expired_wait_reaches_sticky_word(cuda, oracle_lib):
     d = torch.from_numpy(dem).to(cuda)
     b = torch.from_numpy(bc).to(cuda)
     tt.set_tile_shape(2, 2)
+    tt.set_fill_schedule(schedule)
+    try:
+        _expired_wait(tt, _lib, L, d, b, dem, bc, oracle_lib)
+    finally:
+        tt.set_fill_schedule("auto")
+
+
+def _expired_wait(tt, _lib, L, d, b, dem, bc, oracle_lib):
     assert _lib.device_error(clear=True) in (0, 1)  # start clean
     print("forced timeout: fillsinks 1024^2, wait bound 1 tick", flush=True)
     L.tt_set_wait_limit(1)
