@@ -86,14 +86,13 @@ struct FillPrioArgs {
   unsigned max_local;
 };
 
-template <class G>
+template <class G, int MODE>
 __global__ __launch_bounds__(G::THREADS) void fill_prio_persistent(FillPrioArgs a) {
-  __shared__ FillLds<G, kModeFill> lds;
+  __shared__ FillLds<G, MODE> lds;
   const GridTopo topo{a.ntx, a.nty};
   run_prio_queue(a.q, topo, [&](int t, bool first, unsigned nvis, unsigned* lev) -> unsigned {
     const long long x0 = (long long)(t % a.ntx) * G::TX, y0 = (long long)(t / a.ntx) * G::TY;
-    return fill_visit<G, kModeFill>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds,
-                                    lev);
+    return fill_visit<G, MODE>(a.v, x0, y0, first, nvis, a.max_local, a.q.stats_sink(), lds, lev);
   });
 }
 
@@ -112,7 +111,7 @@ __device__ inline unsigned wave_max(unsigned x) {
 // thread reads four adjacent cells per row (16-byte DEM and 4-byte bc
 // loads where the row allows).
 constexpr int kSeedThreads = 256;
-template <class G>
+template <class G, int MODE>
 __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int ntx, PrioQueue q) {
   constexpr int kPerRow = G::TX / 4;              // threads per tile row
   constexpr int kRows = kSeedThreads / kPerRow;   // rows per pass
@@ -120,14 +119,25 @@ __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int
   const int t = blockIdx.x;
   const long long x0 = (long long)(t % ntx) * G::TX, y0 = (long long)(t / ntx) * G::TY;
   const long long gx = x0 + 4 * (threadIdx.x % kPerRow);
+  constexpr bool kRecon = MODE == kModeRecon;
   const bool vec = v.nx % 4 == 0 && gx + 3 < v.nx &&
                    (reinterpret_cast<uintptr_t>(v.src) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(v.bc) & 3) == 0;
+                   (kRecon ? (reinterpret_cast<uintptr_t>(v.state) & 15) == 0
+                           : (reinterpret_cast<uintptr_t>(v.bc) & 3) == 0);
   unsigned smin = ~0u, fmin = ~0u, fmax = 0u;
-  auto cell = [&](float d, unsigned char b) {
+  // fill: d = dem, a seed where bc == 1 at level d; reconstruction (the
+  // negated domain of fill_visit): d = -mask, every cell with a marker a
+  // seed at level max(d, -marker)
+  auto cell = [&](float src, float aux) {
+    const float d = kRecon ? -src : src;
     if (d != d) return;
     const unsigned k = ord_bits(d);
-    if (b == 1) smin = min(smin, k);
+    if (kRecon) {
+      const float f = fmaxf(d, (aux == aux) ? -aux : kInf);
+      if (f < kInf) smin = min(smin, ord_bits(f));
+    } else if (aux == 1.f) {
+      smin = min(smin, k);
+    }
     if (d - d == 0.f) {  // finite
       fmin = min(fmin, k);
       fmax = max(fmax, k);
@@ -140,14 +150,20 @@ __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int
     const size_t g = (size_t)gy * v.nx + gx;
     if (vec) {
       const float4 d = *reinterpret_cast<const float4*>(v.src + g);
-      const uchar4 b = *reinterpret_cast<const uchar4*>(v.bc + g);
-      cell(d.x, b.x);
-      cell(d.y, b.y);
-      cell(d.z, b.z);
-      cell(d.w, b.w);
+      float4 a;
+      if (kRecon) {
+        a = *reinterpret_cast<const float4*>(v.state + g);
+      } else {
+        const uchar4 b = *reinterpret_cast<const uchar4*>(v.bc + g);
+        a = make_float4(b.x, b.y, b.z, b.w);
+      }
+      cell(d.x, a.x);
+      cell(d.y, a.y);
+      cell(d.z, a.z);
+      cell(d.w, a.w);
     } else {
       for (int k = 0; k < 4; k++)
-        if (gx + k < v.nx) cell(v.src[g + k], v.bc[g + k]);
+        if (gx + k < v.nx) cell(v.src[g + k], kRecon ? v.state[g + k] : (float)v.bc[g + k]);
     }
   }
   __shared__ unsigned red[3][kSeedThreads / 64];
@@ -211,7 +227,7 @@ static int launch_blocks(size_t n) {
   return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
-template <class G>
+template <class G, int MODE>
 static hipError_t launch_fill_prio(const FillView& v, hipStream_t stream, RunStats* stats) {
   FillPrioArgs a{};
   a.v = v;
@@ -220,7 +236,8 @@ static hipError_t launch_fill_prio(const FillView& v, hipStream_t stream, RunSta
   a.max_local = 1u << 16;
   const int ntiles = a.ntx * a.nty;
   int grid = 0;
-  hipError_t e = persistent_grid((const void*)fill_prio_persistent<G>, G::THREADS, ntiles, &grid);
+  hipError_t e =
+      persistent_grid((const void*)fill_prio_persistent<G, MODE>, G::THREADS, ntiles, &grid);
   if (e != hipSuccess) return e;
   const size_t words = prio_workspace_words(ntiles);
   void* mem = nullptr;
@@ -242,9 +259,9 @@ static hipError_t launch_fill_prio(const FillView& v, hipStream_t stream, RunSta
   e = sticky_error_word(&a.q.sticky);
   if (e == hipSuccess) {
     zero_words<<<launch_blocks(words), 256, 0, stream>>>(static_cast<unsigned*>(mem), words);
-    prio_seed_kernel<G><<<ntiles, kSeedThreads, 0, stream>>>(v, a.ntx, a.q);
+    prio_seed_kernel<G, MODE><<<ntiles, kSeedThreads, 0, stream>>>(v, a.ntx, a.q);
     prio_queue_kernel<<<launch_blocks((size_t)ntiles), 256, 0, stream>>>(a.q);
-    fill_prio_persistent<G><<<grid, G::THREADS, 0, stream>>>(a);
+    fill_prio_persistent<G, MODE><<<grid, G::THREADS, 0, stream>>>(a);
     e = hipGetLastError();
   }
   if (stats) {
@@ -289,21 +306,22 @@ void set_fill_schedule(FillSchedule s) { g_fill_schedule = s; }
 
 static FillSchedule fill_schedule() { return g_fill_schedule; }
 
-template <class G>
+template <class G, int MODE>
 static hipError_t launch_fill_scheduled(FillArgs a, hipStream_t stream, RunStats* stats) {
   const FillSchedule s = fill_schedule();
   if (s == kSchedAuto) {
+    if (MODE == kModeRecon) return launch_fill<G, MODE>(a, stream, stats);
     const long long ntiles = ((a.v.nx + G::TX - 1) / G::TX) * ((a.v.ny + G::TY - 1) / G::TY);
     int grid = 0;
     hipError_t e =
-        persistent_grid((const void*)fill_prio_persistent<G>, G::THREADS, 1 << 30, &grid);
+        persistent_grid((const void*)fill_prio_persistent<G, MODE>, G::THREADS, 1 << 30, &grid);
     if (e != hipSuccess) return e;
     if (ntiles < (long long)kPrioMinTilesPerWorker * grid || grid > kPrioMaxWorkers)
-      return launch_fill<G, kModeFill>(a, stream, stats);
-    return launch_fill_prio<G>(a.v, stream, stats);
+      return launch_fill<G, MODE>(a, stream, stats);
+    return launch_fill_prio<G, MODE>(a.v, stream, stats);
   }
-  if (s == kSchedFifo) return launch_fill<G, kModeFill>(a, stream, stats);
-  return launch_fill_prio<G>(a.v, stream, stats);
+  if (s == kSchedFifo) return launch_fill<G, MODE>(a, stream, stats);
+  return launch_fill_prio<G, MODE>(a.v, stream, stats);
 }
 
 hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t nx, int64_t ny,
@@ -316,12 +334,21 @@ hipError_t run_fill(float* state, const float* src, const uint8_t* bc, int64_t n
   a.v.nx = nx;
   a.v.ny = ny;
   if (mode == kModeFill) {
-    if (ts.qx == 2 && ts.qy == 2) return launch_fill_scheduled<TileGeom<2, 2>>(a, stream, stats);
-    if (ts.qx == 2 && ts.qy == 1) return launch_fill_scheduled<TileGeom<2, 1>>(a, stream, stats);
-    if (ts.qx == 1 && ts.qy == 2) return launch_fill_scheduled<TileGeom<1, 2>>(a, stream, stats);
-    return launch_fill_scheduled<TileGeom<1, 1>>(a, stream, stats);
+    if (ts.qx == 2 && ts.qy == 2)
+      return launch_fill_scheduled<TileGeom<2, 2>, kModeFill>(a, stream, stats);
+    if (ts.qx == 2 && ts.qy == 1)
+      return launch_fill_scheduled<TileGeom<2, 1>, kModeFill>(a, stream, stats);
+    if (ts.qx == 1 && ts.qy == 2)
+      return launch_fill_scheduled<TileGeom<1, 2>, kModeFill>(a, stream, stats);
+    return launch_fill_scheduled<TileGeom<1, 1>, kModeFill>(a, stream, stats);
   }
-  return dispatch_fill<kModeRecon>(a, ts, stream, stats);
+  if (ts.qx == 2 && ts.qy == 2)
+    return launch_fill_scheduled<TileGeom<2, 2>, kModeRecon>(a, stream, stats);
+  if (ts.qx == 2 && ts.qy == 1)
+    return launch_fill_scheduled<TileGeom<2, 1>, kModeRecon>(a, stream, stats);
+  if (ts.qx == 1 && ts.qy == 2)
+    return launch_fill_scheduled<TileGeom<1, 2>, kModeRecon>(a, stream, stats);
+  return launch_fill_scheduled<TileGeom<1, 1>, kModeRecon>(a, stream, stats);
 }
 
 hipError_t run_fill_capped(float* state, const float* src, const uint8_t* bc, int64_t nx,

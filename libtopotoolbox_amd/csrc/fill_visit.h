@@ -875,8 +875,9 @@ __device__ inline void write_spill_quadrant(const FillView& v, const unsigned lo
 // max_local sweep iterations; kSelf is returned if it stopped there) and
 // writes it back. Returns the neighbours to queue. Called by all threads
 // of the workgroup; `sync` receives the visit statistics. With `lev`
-// (kModeFill; LDS, 8 words), lev[k] is lowered to the order key of the
-// lowest level the tile pushes onto neighbour k.
+// (kModeFill, kModeRecon; LDS, 8 words), lev[k] is lowered to the order
+// key of the lowest level (in the visit's domain) the tile pushes onto
+// neighbour k.
 template <class G, int MODE>
 __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long y0, bool first,
                                       unsigned nvis, unsigned max_local, Sync* sync,
@@ -998,7 +999,7 @@ __device__ inline unsigned fill_visit(const FillView& v, long long x0, long long
                                                               dq = sd[q];
                                                             const T to = L::hi(dq, sf[p]);
                                                             const bool lowers = L::less(to, sf[q]);
-                                                            if constexpr (MODE == kModeFill)
+                                                            if constexpr (MODE != kModeSpill)
                                                               if (lowers && lev)
                                                                 atomicMin(&lev[halo_owner<G>(
                                                                               q / G::P - 1,

@@ -65,8 +65,10 @@ FILL_SCHEDULES = {"auto": 0, "fifo": 1, "priority": 2}
 
 
 def set_fill_schedule(schedule: str) -> None:
-    """Work-queue schedule of single-grid fillsinks for later calls on this
-    thread: "auto" (default), "fifo" or "priority" (same results)."""
+    """Work-queue schedule of single-grid fillsinks and reconstruct for
+    later calls on this thread: "auto" (default), "fifo" or "priority"
+    (same results; "priority" speeds up reconstructions from sparse
+    markers, which "auto" leaves on the FIFO queue)."""
     if schedule not in FILL_SCHEDULES:
         raise ValueError(f"unknown fill schedule {schedule!r}; one of {sorted(FILL_SCHEDULES)}")
     _lib.check(_lib.lib().tt_set_fill_schedule(FILL_SCHEDULES[schedule]), "tt_set_fill_schedule")

@@ -1,7 +1,7 @@
-"""GPU parity of both schedules of the single-grid sink filling: the
-lowest-level-first queue and the FIFO queue each reach the oracle's fixed
-point bit for bit, on every tile shape, whatever the automatic choice
-would have been for the grid size (tt_set_fill_schedule forces one)."""
+"""GPU parity of both schedules of single-grid sink filling and
+reconstruction: the lowest-level-first queue and the FIFO queue each reach
+the oracle's fixed point bit for bit, on every tile shape, whatever the
+automatic choice would have been (tt_set_fill_schedule forces one)."""
 import numpy as np
 import pytest
 import torch
@@ -113,4 +113,34 @@ def test_more_tiles_than_one_bitmap_stretch(cuda, oracle_lib, schedule):
     out, st = fill(dem, bc, (1, 1))
     assert st["tile_x"] == 64 and st["tile_y"] == 64
     assert_bitwise(out, want, f"4160x4170 {schedule} tile (1, 1)")
+
+
+@pytest.mark.parametrize("kind", ["sparse", "dense"])
+def test_reconstruct_schedules(cuda, oracle_lib, schedule, kind):
+    """Reconstruction by dilation on both queues and every tile shape,
+    against the oracle's reconstruct: a sparse marker (the mask on the
+    edge, -INF elsewhere: fillsinks posed as a reconstruction) and a dense
+    one (mask - 5 everywhere, with raised cells), over a mask with NaN
+    holes."""
+    import libtopotoolbox_amd as tt
+
+    rng = np.random.default_rng(41)
+    shape = (700, 650)
+    mask = oracle_lib.grid_random(88, shape)
+    mask[rng.uniform(size=shape) < 0.02] = np.nan
+    if kind == "sparse":
+        marker = np.where(edge_bc(shape) == 1, mask, -np.inf).astype(np.float32)
+    else:
+        marker = (mask - 5.0).astype(np.float32)
+        raised = rng.uniform(size=shape) < 0.01
+        marker[raised] = mask[raised] + 3.0
+    want, _, conv = oracle_lib.reconstruct(marker, mask)
+    assert conv
+    m = torch.from_numpy(marker).cuda()
+    k = torch.from_numpy(mask).cuda()
+    for tile in TILES:
+        tt.set_tile_shape(*tile)
+        out, st = tt.reconstruct(m, k, stats=True)
+        assert st["error"] == 0, st
+        assert_bitwise(out.cpu().numpy(), want, f"reconstruct {kind} {schedule} tile {tile}")
 
