@@ -105,11 +105,12 @@ __device__ inline unsigned wave_max(unsigned x) {
   return x;
 }
 
-// One workgroup per tile: the key of the tile's lowest seed (boundary cell
-// with a value) into q.seed, and the finite DEM range into the control
-// block (pad[0] = ~key(min), pad[1] = key(max), raised from zero). Each
-// thread reads four adjacent cells per row (16-byte DEM and 4-byte bc
-// loads where the row allows).
+// One workgroup per tile: the key of the tile's lowest seed into q.seed,
+// and the finite range of the bounds into the control block (pad[0] =
+// ~key(min), pad[1] = key(max), raised from zero). Fill: a seed is a bc
+// cell with a value, at its DEM level; reconstruction: any cell with a
+// marker, at max(-mask, -marker). Each thread reads four adjacent cells
+// per row (16-byte value and 4-byte bc loads where the row allows).
 constexpr int kSeedThreads = 256;
 template <class G, int MODE>
 __global__ __launch_bounds__(kSeedThreads) void prio_seed_kernel(FillView v, int ntx, PrioQueue q) {

@@ -1,5 +1,6 @@
-// Lowest-level-first scheduling of the persistent tile relaxation (sink
-// filling on one grid).
+// Lowest-level-first scheduling of the persistent tile relaxation of one
+// grid: sink filling, and reconstruction on request (levels in its
+// negated domain, seeds at the marker cells).
 //
 // The FIFO queue of tile_engine.h visits tiles in the order their halos
 // changed. Sink filling lowers levels from the seeds upwards, like a
@@ -24,8 +25,9 @@
 // `window` of them at random, lower ranks more likely (window = twice the
 // number of workgroups: with strictly lowest-first pops, hundreds of idle
 // workgroups race for the same few tiles), clears it and claims the tile
-// if its state still says "queued in this bucket". The counts may lag the bitmaps for a moment;
-// they agree again whenever no push or pop is in flight.
+// if its state still says "queued in this bucket". The counts may lag the
+// bitmaps for a moment; they agree again whenever no push or pop is in
+// flight.
 //
 // Termination and ordering follow tile_engine.h: a pending counter of
 // queued-or-running tiles (zero exactly at convergence), an acquire fence
@@ -39,9 +41,10 @@ namespace tt {
 constexpr int kPrioBuckets = 1024;
 constexpr int kPrioUnreached = kPrioBuckets - 1;
 
-// Queue state of one launch (device memory). The finite DEM range is kept
-// in the control block's spare words as order keys (ord_bits), each raised
-// with atomicMax from zero: pad[0] = ~key(min), pad[1] = key(max).
+// Queue state of one launch (device memory). The finite range of the
+// bounds (the DEM; the negated mask for reconstruction) is kept in the
+// control block's spare words as order keys (ord_bits), each raised with
+// atomicMax from zero: pad[0] = ~key(min), pad[1] = key(max).
 struct PrioQueue {
   Sync* sync;
   int* count;                // per bucket: set bits (may lag for a moment)
@@ -113,7 +116,8 @@ __device__ inline void pq_mark(const PrioQueue& q, int t, int b) {
   const unsigned long long m = 1ull << (t & 63);
   const unsigned long long old =
       __hip_atomic_fetch_or(w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!(old & m)) __hip_atomic_fetch_add(&q.count[b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!(old & m))
+    __hip_atomic_fetch_add(&q.count[b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Queues tile t at bucket b, or moves it down to b (called after a release
