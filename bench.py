@@ -17,8 +17,9 @@ algorithmic bytes (fill 9 B/cell: dem + bc + output; fsm 12 B/cell: dem +
 slopes + output) and 8 TB/s; `cpu_baseline` times the reference path on
 the host (one core) on a bounded crop of the same DEM. The other workload
 is measured too and reported under "extra", with the forward+adjoint
-passes, the batched launch and the stencils next to the path
-(identifyflats, min / max filters).
+passes, reconstruct from a sparse and a dense marker on both work queues,
+the batched launch and the stencils next to the path (identifyflats,
+min / max filters).
 """
 from __future__ import annotations
 
@@ -266,6 +267,64 @@ def batched_run(op: str, count: int, steps: int, cpu: bool = True):
 # square filter's contract also writes tmp (the pass along dims[0]).
 # gwdt_computecosts reads flats, original and filled DEMs and writes float
 # costs and int64 labels.
+def reconstruct_run(arrays: dict, steps: int):
+    """reconstruct (by dilation) at n x n on the fill workload's DEM, from a
+    sparse marker (the negated DEM on the bc cells, -INF elsewhere: fillsinks
+    posed as a reconstruction) and a dense one (marker = DEM - 50 under mask =
+    DEM), each on the FIFO queue (what AUTO runs) and the lowest-level-first
+    queue: ms per call from HIP events (the marker copy the call starts from
+    is outside the timed region), the roofline against 12 B/cell (marker read
+    and written, mask read), and whether the two queues' outputs are
+    bit-identical."""
+    import torch
+
+    import libtopotoolbox_amd as tt
+
+    dev = torch.device("cuda:0")
+    dem = torch.from_numpy(arrays["dem"]).to(dev)
+    bc = torch.from_numpy(arrays["bc"]).to(dev)
+    n = dem.numel()
+    cases = {
+        "sparse_marker": (torch.where(bc == 1, -dem, torch.full_like(dem, -float("inf"))), -dem),
+        "dense_marker": (dem - 50.0, dem),
+    }
+    res = {}
+    for name, (marker, mask) in cases.items():
+        work = torch.empty_like(marker)
+        r, outs = {}, {}
+        for sched in ("fifo", "priority"):
+            tt.set_fill_schedule(sched)
+            try:
+                def call():
+                    work.copy_(marker)
+                    tt.reconstruct(work, mask, inplace=True)
+
+                call()
+                stream = torch.cuda.current_stream()
+                ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                      for _ in range(steps)]
+                for a, b in ev:
+                    work.copy_(marker)
+                    a.record(stream)
+                    tt.reconstruct(work, mask, inplace=True)
+                    b.record(stream)
+                torch.cuda.synchronize()
+                ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+                outs[sched] = work.clone()
+            finally:
+                tt.set_fill_schedule("auto")
+            gbs = 12 * n / (ms * 1e-3) / 1e9
+            r[sched] = {"ms": round(ms, 4), "gcells_per_s": round(n / (ms * 1e-3) / 1e9, 3),
+                        "roofline": {"bound": "hbm", "achieved": round(gbs, 1),
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": 12}}
+        r["auto_runs"] = "fifo"
+        r["queues_bit_identical"] = bool(torch.equal(outs["fifo"].view(torch.int32),
+                                                     outs["priority"].view(torch.int32)))
+        res[name] = r
+    return res
+
+
 STENCIL_BYTES = {"identifyflats": 8, "min_filter_3x3": 8, "max_filter_square_5": 12,
                  "gwdt_computecosts": 24}
 
@@ -716,6 +775,10 @@ def main():
                                                                launch_ms)}
         print(f"[bench] fwd+adjoint {args.workload}: {extra['fwd_adjoint'][args.workload]}",
               file=sys.stderr, flush=True)
+    if args.workload == "fill" and not args.no_extra:
+        extra["reconstruct"] = reconstruct_run(arrays, max(3, args.steps // 2))
+        print(f"[bench] reconstruct: {json.dumps(extra['reconstruct'])}", file=sys.stderr,
+              flush=True)
     del arrays
     if not args.no_extra:
         a2 = make_inputs(other, n)
