@@ -144,3 +144,34 @@ def test_reconstruct_schedules(cuda, oracle_lib, schedule, kind):
         assert st["error"] == 0, st
         assert_bitwise(out.cpu().numpy(), want, f"reconstruct {kind} {schedule} tile {tile}")
 
+
+def test_random_configurations(cuda, oracle_lib, schedule):
+    """Twelve seeded random configurations: sides in [60, 1500], every tile
+    shape, edge / scattered / interior-only boundary cells, NaN holes, -INF
+    and +INF cells, quantised levels (plateaus), against the oracle's
+    restatement of the reference fillsinks."""
+    rng = np.random.default_rng(2027)
+    for k in range(12):
+        ny, nx = (int(v) for v in rng.integers(60, 1500, size=2))
+        dem = oracle_lib.grid_random(500 + k, (ny, nx))
+        if k % 3 == 0:
+            dem = np.floor(dem * 4.0) / 4.0  # plateaus
+        u = rng.uniform(size=dem.shape)
+        dem[u < 0.01] = np.nan
+        dem[(u >= 0.01) & (u < 0.012)] = -np.inf
+        dem[(u >= 0.012) & (u < 0.014)] = np.inf
+        dem = dem.astype(np.float32)
+        kind = k % 3
+        if kind == 0:
+            bc = edge_bc(dem.shape)
+        elif kind == 1:
+            bc = (rng.uniform(size=dem.shape) < 0.002).astype(np.uint8)
+        else:
+            bc = np.zeros(dem.shape, np.uint8)
+            bc[ny // 3:ny // 3 + 3, nx // 2] = 1
+        want, _, conv = oracle_lib.fillsinks(dem, bc)
+        assert conv
+        tile = TILES[k % len(TILES)]
+        out, _ = fill(dem, bc, tile)
+        assert_bitwise(out, want, f"config {k} ({ny}x{nx}, bc kind {kind}) {schedule} tile {tile}")
+
