@@ -84,6 +84,13 @@ __device__ unsigned find_halving(unsigned* P, unsigned x) {
   }
 }
 
+// Cached load, for guards that tolerate a stale word: a word that only
+// grows (or only shrinks, or only gains bits) read stale only lets an
+// atomic through that was not needed.
+__device__ __forceinline__ unsigned ldc(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Links the sets of a and b, the larger root under the smaller.
 __device__ void unite(unsigned* P, unsigned a, unsigned b) {
   for (;;) {
@@ -179,7 +186,7 @@ __device__ __forceinline__ bool deepest_pixel(unsigned k, float d, unsigned wbit
 struct Ws {
   unsigned* L;   // per pixel: tile root (global index); kNone off flats
   unsigned* G;   // per tile root: union-find across tiles; then the tie keys' positions
-  unsigned* Kc;  // per component: kind bits, kMulti if it spans several tiles
+  unsigned* Kc;  // per component: kind bits
   unsigned* Wc;  // per component: largest depth (order-preserving bits)
   unsigned* Cc;  // per component: number of deepest pixels (at least up to 2)
   unsigned* Fc;  // per component: first deepest pixel, then the label
@@ -194,8 +201,6 @@ struct Ws {
   unsigned ntx, nty;
   unsigned* cnt; // [0] tie keys, [1] consistency errors
 };
-
-constexpr unsigned kMulti = 4u;
 
 // Tile labelling and the tile's share of every reduction: 64 x kTileH
 // pixels per workgroup, one lane per column, wavefront w on rows w, w + 4,
@@ -401,39 +406,26 @@ __global__ __launch_bounds__(128) void gw_border(unsigned nx, unsigned ny, Ws w)
   }
 }
 
-// Loops over the list entries: one workgroup (one wavefront) per tile.
-#define GW_FOR_ENTRIES                                                       \
-  for (unsigned tile = blockIdx.x; tile < w.ntx * w.nty; tile += gridDim.x) \
-    for (unsigned e = tile * w.cap + threadIdx.x, e_end = tile * w.cap + w.TC[tile]; e < e_end; \
-         e += blockDim.x)
+// Loops over the list entries: one wavefront per tile.
+#define GW_FOR_ENTRIES                                                                    \
+  for (unsigned tile = blockIdx.x * 4u + (threadIdx.x >> 6); tile < w.ntx * w.nty;         \
+       tile += gridDim.x * 4u)                                                            \
+    for (unsigned e = tile * w.cap + (threadIdx.x & 63u), e_end = tile * w.cap + w.TC[tile]; \
+         e < e_end; e += 64u)
 
 // Component of every tile root (into RH: a root stored into G itself could
 // be overwritten by another thread's path halving that read its old parent
-// just before); components with more than one tile root are marked.
-__global__ __launch_bounds__(64) void gw_roots(Ws w) {
+// just before), and the component's kind bits and largest depth from its
+// tile roots' shares. Each atomic is issued only where it changes the word
+// (a large lake has thousands of tile roots, and atomics on one word queue
+// up).
+__global__ __launch_bounds__(256) void gw_roots(Ws w) {
   GW_FOR_ENTRIES {
     const unsigned r = w.RL[e], c = find_halving(w.G, r);
     w.RH[e] = c;
-    if (c != r && !(ld(w.Kc + c) & kMulti)) atomicOr(w.Kc + c, kMulti);
-  }
-}
-
-// Kind bits and largest depth per component: the entry's own values for a
-// component of one tile root, atomics over its tile roots otherwise, each
-// issued only where it changes the word (a large lake has thousands of tile
-// roots, and atomics on one word queue up).
-__global__ __launch_bounds__(64) void gw_merge_depth(Ws w) {
-  GW_FOR_ENTRIES {
-    const unsigned c = w.RH[e];
-    const unsigned kc = ld(w.Kc + c);
-    if (kc & kMulti) {
-      const unsigned k = w.RK[e], wb = w.RW[e];
-      if ((kc & k) != k) atomicOr(w.Kc + c, k);
-      if (wb > ld(w.Wc + c)) atomicMax(w.Wc + c, wb);
-    } else {
-      w.Kc[c] = w.RK[e];
-      w.Wc[c] = w.RW[e];
-    }
+    const unsigned k = w.RK[e], wb = w.RW[e];
+    if ((ldc(w.Kc + c) & k) != k) atomicOr(w.Kc + c, k);
+    if (wb > ldc(w.Wc + c)) atomicMax(w.Wc + c, wb);
   }
 }
 
@@ -441,19 +433,13 @@ __global__ __launch_bounds__(64) void gw_merge_depth(Ws w) {
 // more matters) and the first of them per component, from the tile roots
 // whose pixels reach the component's largest depth; guarded atomics as
 // above.
-__global__ __launch_bounds__(64) void gw_merge_deepest(Ws w) {
+__global__ __launch_bounds__(256) void gw_merge_deepest(Ws w) {
   GW_FOR_ENTRIES {
-    const unsigned c = w.RH[e], kc = w.Kc[c];
-    const unsigned kind = kc & 3u;
+    const unsigned c = w.RH[e], kind = w.Kc[c];
     if (kind == kMixed) continue;
     if (kind == kHasNumber && w.RW[e] != w.Wc[c]) continue;
-    if (kc & kMulti) {
-      if (ld(w.Cc + c) < 2u) atomicAdd(w.Cc + c, w.RC[e]);
-      if (w.RF[e] < ld(w.Fc + c)) atomicMin(w.Fc + c, w.RF[e]);
-    } else {
-      w.Cc[c] = w.RC[e];
-      w.Fc[c] = w.RF[e];
-    }
+    if (ldc(w.Cc + c) < 2u) atomicAdd(w.Cc + c, w.RC[e]);
+    if (w.RF[e] < ldc(w.Fc + c)) atomicMin(w.Fc + c, w.RF[e]);
   }
 }
 
@@ -471,7 +457,7 @@ __device__ __forceinline__ float gw_cost(float label_depth, float depth) {
 __global__ __launch_bounds__(256) void gw_final(const float* original, const float* filled,
                                                 unsigned nx, unsigned ny, Ws w, float* costs,
                                                 int64_t* conncomps, unsigned long long* keys,
-                                                unsigned cb) {
+                                                unsigned* parent, unsigned cb) {
   __shared__ unsigned sComp[kTileH * kTileW], sLab[kTileH * kTileW];
   __shared__ float sDep[kTileH * kTileW];
   const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -497,7 +483,7 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
          e += blockDim.x) {
       const unsigned r = w.RL[e], c = w.RH[e];
       const unsigned slot = (r / nx - y0) * kTileW + (r % nx - x0);
-      const bool tie = (w.Kc[c] & 3u) == kMixed || w.Cc[c] >= 2u;
+      const bool tie = w.Kc[c] == kMixed || w.Cc[c] >= 2u;
       const unsigned label = tie ? kNone : w.Fc[c];
       sComp[slot] = c;
       sLab[slot] = label;
@@ -505,11 +491,13 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
     }
     __syncthreads();
     unsigned long long tm[R];
+    unsigned par[R];
     unsigned ties = 0u;
 #pragma unroll
     for (unsigned k = 0; k < R; k++) {
       const unsigned y = y0 + wave + 4 * k;
       bool tie = false;
+      par[k] = kNone;
       if (x < nx && y < ny) {
         const unsigned p = y * nx + x, l = lv[k];
         if (l == kNone) {
@@ -521,7 +509,7 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
           if (label < n) {
             costs[p] = gw_cost(sDep[slot], dv[k]);
             conncomps[p] = (int64_t)label;
-          } else if ((w.Kc[sComp[slot]] & 3u) == kMixed || w.Cc[sComp[slot]] >= 2u) {
+          } else if (w.Kc[sComp[slot]] == kMixed || w.Cc[sComp[slot]] >= 2u) {
             tie = true;
           } else {
             atomicOr(w.cnt + 1, 1u);  // no deepest pixel: cannot happen
@@ -530,6 +518,15 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
       }
       tm[k] = __ballot(tie);
       ties += (unsigned)__popcll(tm[k]);
+      // chain parent of a tie pixel: the start of its run in this tile row,
+      // or at the start, its first flat scanned neighbour (itself if none)
+      const unsigned long long fb = __ballot(lv[k] != kNone);
+      if (tie) {
+        const unsigned long long left = ~fb & ((1ull << lane) - 1ull);
+        const unsigned st = left ? 64u - (unsigned)__clzll((long long)left) : 0u;
+        const unsigned p = y * nx + x;
+        par[k] = st < lane ? p - (lane - st) : scanned_parent(Grid{w.L, nx, ny}, x, y);
+      }
     }
     if (ties) {
       unsigned base = 0u;
@@ -540,8 +537,9 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
         if ((tm[k] >> lane) & 1ull) {
           const unsigned p = (y0 + wave + 4 * k) * nx + x;
           const unsigned slot = (lv[k] / nx - y0) * kTileW + (lv[k] % nx - x0);
-          keys[base + (unsigned)__popcll(tm[k] & ((1ull << lane) - 1ull))] =
-              ((unsigned long long)sComp[slot] << cb) | (unsigned long long)p;
+          const unsigned q = base + (unsigned)__popcll(tm[k] & ((1ull << lane) - 1ull));
+          keys[q] = ((unsigned long long)sComp[slot] << cb) | (unsigned long long)p;
+          parent[q] = par[k];
         }
         base += (unsigned)__popcll(tm[k]);
       }
@@ -556,10 +554,11 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
   for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < (n); q += gridDim.x * blockDim.x)
 
 struct Tie {
-  const unsigned long long* keys;  // sorted: component << cb | pixel
+  const unsigned long long* keys;  // component << cb | pixel, in no particular order
   unsigned nk;
   unsigned n;      // pixels
   unsigned cb;     // bits of a pixel index in a key
+  const unsigned* parent;  // per key: its chain parent pixel (itself at a birth)
   unsigned* P;    // per pixel: position of its key (the G array)
   unsigned* B;    // per key: chain parent, as a position
   unsigned* R;    // per key: its birth, as a position
@@ -589,23 +588,22 @@ __device__ __forceinline__ unsigned key_pos(const Tie& t, unsigned p, unsigned* 
   return kNone;
 }
 
-// Chain parent of every key (a flat scanned neighbour lies in the same
-// component, so its key precedes); a position that does not precede marks
-// an inconsistency and is not followed.
-__global__ void gw_tie_parent(Tie t, Grid g, const unsigned* K, unsigned* err) {
+// Chain parent of every key, as a position: the parent pixel is a flat of
+// the same component with a smaller index (an earlier pixel of its run, or
+// a flat scanned neighbour), so it has a key and the chains end at births
+// whatever the order of the keys; a pixel without a key marks an
+// inconsistency and is not followed.
+__global__ void gw_tie_parent(Tie t, const unsigned* K, unsigned* err) {
   GW_FOR_KEYS(t.nk) {
     const unsigned long long key = t.keys[q];
     const unsigned p = key_pixel(key, t.cb);
     t.D[q] = kNone;
     unsigned b = q;
-    if ((K[key_comp(key, t.cb)] & 3u) != kMixed) {
-      const unsigned par = scanned_parent(g, p % g.nx, p / g.nx);
-      if (par != p) {
+    if (K[key_comp(key, t.cb)] != kMixed) {
+      const unsigned par = t.parent[q];
+      if (par < p) {
         b = key_pos(t, par, err);
-        if (b >= q) {
-          atomicOr(err, 1u);
-          b = q;
-        }
+        if (b == kNone) b = q;
       }
     }
     t.B[q] = b;
@@ -614,10 +612,9 @@ __global__ void gw_tie_parent(Tie t, Grid g, const unsigned* K, unsigned* err) {
 
 __global__ void gw_tie_births(Tie t) { GW_FOR_KEYS(t.nk) t.R[q] = find_halving(t.B, q); }
 
-// First deepest pixel of every chain group. Keys are in scan order, so a
-// wavefront's lanes of one group have their smallest pixel in the lowest
-// lane: one guarded atomic per group and wavefront (a large all-NaN flat
-// is one group of millions of deepest pixels).
+// First deepest pixel of every chain group: per wavefront, one guarded
+// atomic per group present (a large all-NaN flat is one group of millions
+// of deepest pixels).
 __global__ void gw_tie_deepest(Tie t, const unsigned* K, const unsigned* W, const float* original,
                                const float* filled) {
   const unsigned lane = threadIdx.x & 63u;
@@ -627,7 +624,7 @@ __global__ void gw_tie_deepest(Tie t, const unsigned* K, const unsigned* W, cons
     unsigned b = kNone, p = kNone;
     if (q < t.nk) {
       const unsigned long long key = t.keys[q];
-      const unsigned c = key_comp(key, t.cb), k = K[c] & 3u;
+      const unsigned c = key_comp(key, t.cb), k = K[c];
       p = key_pixel(key, t.cb);
       if (k != kMixed && deepest_pixel(k, depth_at(original, filled, p), W[c])) b = t.R[q];
     }
@@ -635,8 +632,11 @@ __global__ void gw_tie_deepest(Tie t, const unsigned* K, const unsigned* W, cons
     while (pending) {
       const int leader = __ffsll((long long)pending) - 1;
       const unsigned b0 = __shfl(b, leader);
-      if ((int)lane == leader && p < ld(t.D + b0)) atomicMin(t.D + b0, p);
-      pending &= ~__ballot(b == b0);
+      const bool mine = b == b0;
+      unsigned m = mine ? p : kNone;
+      for (int dd = 32; dd > 0; dd >>= 1) m = min(m, __shfl_xor(m, dd));
+      if ((int)lane == leader && m < ldc(t.D + b0)) atomicMin(t.D + b0, m);
+      pending &= ~__ballot(mine);
     }
   }
 }
@@ -647,7 +647,7 @@ __global__ void gw_tie_flags(Tie t, Grid g, const unsigned* K, unsigned* err) {
   GW_FOR_KEYS(t.nk) {
     const unsigned long long key = t.keys[q];
     const unsigned p = key_pixel(key, t.cb);
-    bool ev = (K[key_comp(key, t.cb)] & 3u) == kMixed;
+    bool ev = K[key_comp(key, t.cb)] == kMixed;
     if (!ev) {
       const unsigned b = t.R[q];
       ev = b == q || t.D[b] == p;
@@ -727,7 +727,7 @@ __global__ void gw_tie_replay(Tie t, Grid g, const unsigned long long* ev, const
     if (q > 0 && key_comp(ev[q - 1], t.cb) == c) continue;  // not the first of its component
     unsigned e = q;
     while (e < n && key_comp(ev[e], t.cb) == c) e++;
-    const unsigned kind = K[c] & 3u;
+    const unsigned kind = K[c];
     if (kind == kMixed) {
       replay_full(t, g, ev, q, e, original, filled, Mf, err);
       continue;
@@ -823,8 +823,8 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   const unsigned cap = (kTileW / 2) * ((tile_rows + 1) / 2);
   const size_t ne = ntiles * cap;
   // per pixel six words, per list entry six, per tile one, counters, and up
-  // to one tie key per pixel, in one workspace
-  const size_t words = 6 * n + 6 * ne + ntiles + 16;
+  // to one tie key (and its chain parent) per pixel, in one workspace
+  const size_t words = 7 * n + 6 * ne + ntiles + 16;
   const size_t keys_off = up256(words * sizeof(unsigned));
   unsigned* ws = nullptr;
   hipError_t e = hipMallocAsync((void**)&ws, keys_off + n * sizeof(unsigned long long), s);
@@ -850,6 +850,7 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   w.RF = take(ne);
   w.TC = take(ntiles);
   w.cnt = take(16);  // [0] tie keys, [1] consistency errors
+  unsigned* parent = take(n);
   w.cap = cap;
   w.ntx = ntx;
   w.nty = nty;
@@ -861,14 +862,14 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
     return r != hipSuccess ? r : f;
   };
   const dim3 tgrid(ntx, capped(nty, 65535));
-  const unsigned egrid = capped(ntiles, 1u << 20);
+  const unsigned egrid = capped((ntiles + 3) / 4, 1u << 20);
   gw_zero<<<1, 64, 0, s>>>(w.cnt, 16);
   gw_local<<<tgrid, 256, 0, s>>>(flats, original, filled, nx, ny, w);
   gw_border<<<tgrid, 128, 0, s>>>(nx, ny, w);
-  gw_roots<<<egrid, 64, 0, s>>>(w);
-  gw_merge_depth<<<egrid, 64, 0, s>>>(w);
-  gw_merge_deepest<<<egrid, 64, 0, s>>>(w);
-  gw_final<<<tgrid, 256, 0, s>>>(original, filled, nx, ny, w, costs, conncomps, keys, cb);
+  gw_roots<<<egrid, 256, 0, s>>>(w);
+  gw_merge_deepest<<<egrid, 256, 0, s>>>(w);
+  gw_final<<<tgrid, 256, 0, s>>>(original, filled, nx, ny, w, costs, conncomps, keys, parent,
+                                 cb);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
   unsigned host[2] = {0, 0};
   e = hipMemcpyAsync(host, w.cnt, sizeof(host), hipMemcpyDeviceToHost, s);
@@ -879,8 +880,9 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   if (host[0] > 0x7fffffffu) return done(hipErrorInvalidValue);  // hipcub counts in int
   const unsigned nk = host[0];
 
-  // Tie components: keys sorted (component, then pixel), chain groups,
-  // the replay and their outputs.
+  // Tie components: chain groups over their keys (in append order), then
+  // the replay over the scan steps it needs, sorted (component, then
+  // pixel), and their outputs.
   size_t sort_bytes = 0, flag_bytes = 0;
   e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (unsigned long long*)nullptr,
                                         (unsigned long long*)nullptr, (int)nk, 0, (int)(2 * cb),
@@ -898,11 +900,12 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   if (e != hipSuccess) return done(e);
   char* m = (char*)tie_ws;
   Tie t;
-  unsigned long long* sorted = (unsigned long long*)m;
+  unsigned long long* ev = (unsigned long long*)m;  // the flagged keys
   m += key_bytes;
-  unsigned long long* ev = (unsigned long long*)m;
+  unsigned long long* evs = (unsigned long long*)m;  // the same, sorted
   m += key_bytes;
-  t.keys = sorted;
+  t.keys = keys;
+  t.parent = parent;
   t.nk = nk;
   t.n = (unsigned)n;
   t.cb = cb;
@@ -922,21 +925,27 @@ hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t
   unsigned* nev = (unsigned*)m;
   m += 256;
   void* temp = m;
-  size_t tb = sort_bytes;
-  e = hipcub::DeviceRadixSort::SortKeys(temp, tb, keys, sorted, (int)nk, 0, (int)(2 * cb), s);
-  if (e != hipSuccess) return done(e);
   const Grid g{w.L, nx, ny};
   unsigned* err = w.cnt + 1;
   const unsigned kb = capped((nk + 255) / 256, 16384);
   gw_tie_pos<<<kb, 256, 0, s>>>(t);
-  gw_tie_parent<<<kb, 256, 0, s>>>(t, g, w.Kc, err);
+  gw_tie_parent<<<kb, 256, 0, s>>>(t, w.Kc, err);
   gw_tie_births<<<kb, 256, 0, s>>>(t);
   gw_tie_deepest<<<kb, 256, 0, s>>>(t, w.Kc, w.Wc, original, filled);
   gw_tie_flags<<<kb, 256, 0, s>>>(t, g, w.Kc, err);
-  tb = flag_bytes;
-  e = hipcub::DeviceSelect::Flagged(temp, tb, sorted, t.flag, ev, nev, (int)nk, s);
+  size_t tb = flag_bytes;
+  e = hipcub::DeviceSelect::Flagged(temp, tb, keys, t.flag, ev, nev, (int)nk, s);
+  unsigned nev_host = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&nev_host, nev, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return done(e);
-  gw_tie_replay<<<kb, 256, 0, s>>>(t, g, ev, nev, w.Kc, w.Wc, w.Fc, original, filled, err);
+  if (nev_host) {
+    tb = sort_bytes;
+    e = hipcub::DeviceRadixSort::SortKeys(temp, tb, ev, evs, (int)nev_host, 0, (int)(2 * cb), s);
+    if (e != hipSuccess) return done(e);
+    gw_tie_replay<<<capped((nev_host + 255) / 256, 16384), 256, 0, s>>>(
+        t, g, evs, nev, w.Kc, w.Wc, w.Fc, original, filled, err);
+  }
   gw_tie_outputs<<<kb, 256, 0, s>>>(t, w.Fc, (unsigned)n, original, filled, costs, conncomps,
                                     err);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
