@@ -268,9 +268,11 @@ int tt_identifyflats_dev(int32_t *output, const float *dem, ptrdiff_t nx, ptrdif
                          void *stream, int64_t *count);
 
 /* gwdt_computecosts on device buffers. Waits for completion (the number
-   of tie-breaking steps to replay is read back once). NaN depths are
-   handled as in the reference (see gwdt_computecosts). Returns nonzero
-   with a message for nx * ny >= 2^32. */
+   of pixels in components whose deepest depth is tied is read back, and
+   when there are any, the number of scan steps to replay). NaN depths are
+   handled as in the reference (see gwdt_computecosts). Takes about 42
+   bytes of device workspace per pixel. Returns nonzero with a message for
+   nx * ny >= 2^32. */
 int tt_gwdt_computecosts_dev(float *costs, int64_t *conncomps, const int32_t *flats,
                              const float *original_dem, const float *filled_dem, ptrdiff_t nx,
                              ptrdiff_t ny, void *stream);

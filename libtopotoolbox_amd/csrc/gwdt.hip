@@ -10,18 +10,19 @@
 // The label is that final root: a pixel of largest depth, and when several
 // pixels share it, the one the scan's merge order picks.
 //
-// Components come from a tile-parallel union-find: each 64 x 32 tile is
+// Components come from a tile-parallel union-find: each 64 x 16 tile is
 // labelled in LDS, the tiles are joined across their borders in global
 // memory, and every flat then learns its component, whose representative
 // is its first pixel in scan order. The largest depth of each component,
-// the number of pixels that reach it and the first of them follow from two
-// reductions. A component with one pixel of largest depth is labelled with
-// it, and the last pass writes its costs and labels. The other components
-// ("tie" components: two or more deepest pixels, or depths mixing NaN and
-// numbers) have all their pixels collected as keys component << 32 | pixel
-// and sorted, and their label is replayed from the few scan steps that can
-// change which deepest pixel is the root, in scan order, one component per
-// thread:
+// the number of pixels that reach it and the first of them are reduced per
+// tile root in LDS, then over the tile roots of each component. A
+// component with one pixel of largest depth is labelled with it, and the
+// last pass writes its costs and labels. The other components ("tie"
+// components: two or more deepest pixels, or depths mixing NaN and
+// numbers) have all their pixels collected as keys component << cb |
+// pixel, and their label is replayed from the few scan steps that can
+// change which deepest pixel is the root, sorted into scan order, one
+// component per thread:
 //   birth     a flat with no flat scanned neighbour starts a set;
 //   merge     a flat whose scanned neighbours lie in two different sets
 //             (only possible when (i,j-1) is not flat, (i+1,j-1) is, and
@@ -34,7 +35,7 @@
 // strictly deeper pixel can displace a root. Sets are tracked by their
 // birth pixel: each flat points at one flat scanned neighbour (its chain
 // parent), and the chains end at births; the chains are resolved over the
-// sorted keys of the tie components only.
+// keys of the tie components only.
 //
 // NaN depths (e.g. flats of -INF cells, where filled - original is NaN):
 // the scan's "strictly deeper" test is false whenever a NaN is involved, so

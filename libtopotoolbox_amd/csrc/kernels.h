@@ -145,7 +145,8 @@ hipError_t run_value_filter(int op, float* out, const float* dem, const uint8_t*
 hipError_t run_value_filter_square(int op, float* out, const float* dem, float* tmp, int width,
                                    int64_t nx, int64_t ny, hipStream_t stream);
 
-// gwdt_computecosts (gwdt.hip); synchronises the stream once.
+// gwdt_computecosts (gwdt.hip); synchronises the stream (once, or three times
+// when components with tied deepest depths need the scan replay).
 hipError_t run_gwdt_computecosts(float* costs, int64_t* conncomps, const int32_t* flats,
                                  const float* original, const float* filled, int64_t nx,
                                  int64_t ny, hipStream_t stream);

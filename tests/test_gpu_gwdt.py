@@ -143,3 +143,47 @@ def test_host_dropin(oracle_lib):
     got_c, got_cc = tt.gwdt_computecosts_host(f, o, fl)
     np.testing.assert_array_equal(got_cc, want_cc)
     assert_costs(got_c, want_c, "host drop-in")
+
+
+@pytest.mark.parametrize("pattern", ["diag", "anti", "checker", "hstripes", "vstripes", "stairs"])
+def test_tile_border_patterns(oracle_lib, pattern):
+    """Flats that cross the 64 x 16 labelling tiles in every direction
+    (single-pixel diagonal chains, checkerboards, stripes, staircases), on a
+    grid that is no multiple of the tile, with tied and distinct depths."""
+    rng = np.random.default_rng(hash(pattern) % 1000)
+    ny, nx = 211, 301
+    y, x = np.mgrid[0:ny, 0:nx]
+    flats = {
+        "diag": (x - y) % 9 == 0,
+        "anti": (x + y) % 7 == 0,
+        "checker": (x + y) % 2 == 0,
+        "hstripes": (y % 17) < 15,
+        "vstripes": (x % 65) < 62,
+        "stairs": ((x // 5 + y // 3) % 4) != 0,
+    }[pattern].astype(np.int32)
+    flats[0, :] = flats[-1, :] = 0
+    flats[:, 0] = flats[:, -1] = 0
+    orig = rng.uniform(0, 10, size=(ny, nx)).astype(np.float32)
+    for levels in (1, 3, 50):
+        depth = rng.integers(0, levels, size=(ny, nx)).astype(np.float32)
+        check(flats, orig, (orig + depth).astype(np.float32), oracle_lib,
+              f"{pattern} levels {levels}")
+
+
+def test_many_tile_ties_and_nan_flats(oracle_lib):
+    """Components spanning many tiles whose deepest depth is shared (two
+    depth levels), next to a large all-NaN flat (INF - INF depths) and
+    components mixing NaN and numbers: the tie replay over many tiles."""
+    rng = np.random.default_rng(77)
+    ny, nx = 700, 900
+    flats = (rng.uniform(size=(ny, nx)) < 0.9).astype(np.int32)
+    flats[0, :] = flats[-1, :] = 0
+    flats[:, 0] = flats[:, -1] = 0
+    orig = rng.uniform(0, 10, size=(ny, nx)).astype(np.float32)
+    filled = (orig + rng.integers(0, 2, size=(ny, nx))).astype(np.float32)
+    y, x = np.mgrid[0:ny, 0:nx]
+    blob = np.hypot(y - 350, x - 600) < 180
+    orig[blob] = filled[blob] = np.inf
+    mixed = (np.hypot(y - 150, x - 150) < 60) & (rng.uniform(size=(ny, nx)) < 0.3)
+    orig[mixed] = filled[mixed] = -np.inf
+    check(flats, orig, filled, oracle_lib, "many-tile ties")
