@@ -392,6 +392,60 @@ def stencil_run(n: int, steps: int, cpu: bool):
     return res
 
 
+def fmm3d_run(n: int, steps: int, cpu: bool, nlayers: int = 4):
+    """excesstopography_fmm3d on the excess-topography workload's DEM
+    (n x n, cellsize 30 m) with `nlayers` horizontal layers whose tops are
+    equally spaced over the DEM's range, and threshold slopes alternating
+    1.0 / 0.5, as in the reference's own test (test/excesstopography.cpp:
+    84-145). Reports the time per call from HIP events and a roofline on
+    dem + layer tops + output. The compiled reference is timed on a
+    2048 x 2048 crop. Parity for fmm3d is partial (the layered rule is
+    order-dependent), so no parity spot is reported."""
+    import torch
+
+    import libtopotoolbox_amd as tt
+    from libtopotoolbox_amd import synthetic as S
+
+    dem_np = S.dem_pits(n, n, 1)
+    finite = dem_np[np.isfinite(dem_np)]
+    lo, hi = float(finite.min()), float(finite.max())
+    tops = np.float32(lo) + np.float32(hi - lo) * (np.arange(nlayers, dtype=np.float32) + 1) \
+        / np.float32(nlayers)
+    slopes_np = np.where(np.arange(nlayers) % 2 == 0, 1.0, 0.5).astype(np.float32)
+    dev = torch.device("cuda:0")
+    dem = torch.from_numpy(dem_np).to(dev)
+    lith = torch.from_numpy(tops).to(dev).expand(n, n, nlayers).contiguous()
+    slopes = torch.from_numpy(slopes_np).to(dev)
+    out = torch.empty_like(dem)
+    call = lambda: tt.excesstopography_fmm3d(dem, lith, slopes, 30.0, out=out)
+    call()
+    _, per = time_events(call, steps)
+    ms = float(np.mean(per))
+    bpc = 8 + 4 * nlayers
+    gbs = bpc * n * n / (ms * 1e-3) / 1e9
+    res = {"ms": round(ms, 4), "gcells_per_s": round(n * n / (ms * 1e-3) / 1e9, 3),
+           "layers": nlayers, "layer_tops_m": [round(float(t), 2) for t in tops],
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "bytes_per_cell": bpc}}
+    if cpu:
+        import oracle
+
+        if oracle.ref() is not None:
+            c = 2048
+            crop = np.ascontiguousarray(dem_np[:c, :c])
+            lith_c = np.ascontiguousarray(np.broadcast_to(tops, (c, c, nlayers)))
+            t0 = time.perf_counter()
+            oracle.ref_fmm3d(crop, lith_c, slopes_np, 30.0)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": round(c * c / dt / 1e9, 5), "unit": "Gcells/s",
+                                   "cores": 1, "kind": "reference", "seconds": round(dt, 2),
+                                   "sample": "2048x2048 crop of the DEM, same layers, one call"}
+    del dem, lith, slopes, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def roofline(workload: str, cells: int, launch_ms: float, traffic):
     algo = BYTES_PER_CELL[workload] * cells
     achieved = algo / (launch_ms * 1e-3) / 1e9
@@ -800,6 +854,8 @@ def main():
     if not args.no_stencils:
         extra["stencils"] = stencil_run(n, max(5, args.steps), not args.no_cpu)
         print(f"[bench] stencils: {json.dumps(extra['stencils'])}", file=sys.stderr, flush=True)
+        extra["fmm3d"] = fmm3d_run(n, max(3, args.steps // 2), not args.no_cpu)
+        print(f"[bench] fmm3d: {json.dumps(extra['fmm3d'])}", file=sys.stderr, flush=True)
     if not args.no_batched:
         import torch
 
