@@ -234,13 +234,44 @@ __global__ __launch_bounds__(kThreads, 5) void identifyflats_kernel(
 }
 
 // identifyflats streaming through registers (nx % 4 == 0, 16-byte aligned
-// arrays), laid out as value_filter_3x3: each lane owns four columns of a
-// kRowsF-row strip. It holds the values of rows -2 .. kRowsF + 1 and
-// columns -2 .. 5 around them (side columns by DPP, the wavefront's
-// outside ones by a load in its edge lanes), computes the flat bits of
-// rows -1 .. kRowsF and columns -1 .. 4 itself, then the flags of its own
-// cells, with the tiled kernel's rules.
-constexpr int kRowsF = 4;
+// arrays): each lane owns four columns, and each wavefront walks a strip of
+// kStripF rows down the grid with a rolling window. It keeps the values of
+// the last four rows (columns -2 .. 5: side columns by DPP, the
+// wavefront's outside ones by a load in its edge lanes), the order keys of
+// the last three and the flat bits of the last three, with the loads of
+// the next kAheadF rows in flight. A row's flat bits need its 3x3 windows
+// and a row's flags the flat bits of the rows around it, so the flags
+// trail the loads by two rows; the rules are the tiled kernel's. Each row
+// is read once per strip, plus four halo rows per strip.
+constexpr int kStripF = 32, kAheadF = 4;
+
+struct RawRowF {
+  float4 c;
+  float2 lw, rw;
+};
+
+__device__ __forceinline__ RawRowF load_row_f(const float* dem, long long nx, long long ny,
+                                              long long x, long long y, int lane, bool xin) {
+  const float nan = __builtin_nanf("");
+  RawRowF r;
+  r.c = make_float4(nan, nan, nan, nan);
+  r.lw = make_float2(nan, nan);
+  r.rw = make_float2(nan, nan);
+  const bool yin = y >= 0 && y < ny;
+  if (xin && yin) r.c = *reinterpret_cast<const float4*>(dem + y * nx + x);
+  if (lane == 0 && yin && x >= 2 && x - 2 < nx)
+    r.lw = *reinterpret_cast<const float2*>(dem + y * nx + x - 2);
+  if (lane == 63 && yin && x + 4 < nx)
+    r.rw = *reinterpret_cast<const float2*>(dem + y * nx + x + 4);
+  return r;
+}
+
+// Order-preserving integer key of a value, NaN as -INF and -0 as +0 (the
+// float comparison finds -0 and +0 equal).
+__device__ __forceinline__ unsigned flat_key(float q) {
+  const unsigned u = __float_as_uint(q + 0.0f);  // -0 + 0 = +0
+  return q != q ? 0x007fffffu : (u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u));
+}
 
 __global__ __launch_bounds__(256) void identifyflats_stream(int32_t* out, const float* dem,
                                                             long long nx, long long ny,
@@ -254,111 +285,124 @@ __global__ __launch_bounds__(256) void identifyflats_stream(int32_t* out, const 
   if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
   const long long bx = L % gxb, by = L / gxb;
   const long long x = (bx * 64 + lane) * 4;
-  const long long y0 = (by * 4 + wave) * kRowsF;
-  const bool active = y0 < ny, xin = x < nx;
+  const long long ys = (by * 4 + wave) * kStripF;
+  const bool xin = x < nx;
   const float nan = __builtin_nanf("");
-  constexpr int NR = kRowsF + 4;
-  float v[NR][8];  // value rows y0 - 2 + i, columns x - 2 + j
-#pragma unroll
-  for (int i = 0; i < NR; i++) {
-    const long long y = y0 - 2 + i;
-    const bool yin = active && y >= 0 && y < ny;
-    float4 c = make_float4(nan, nan, nan, nan);
-    float2 lw = make_float2(nan, nan), rw = make_float2(nan, nan);
-    if (xin && yin) c = *reinterpret_cast<const float4*>(dem + y * nx + x);
-    if (lane == 0 && yin && x >= 2 && x - 2 < nx)
-      lw = *reinterpret_cast<const float2*>(dem + y * nx + x - 2);
-    if (lane == 63 && yin && x + 4 < nx)
-      rw = *reinterpret_cast<const float2*>(dem + y * nx + x + 4);
-    auto shr = [](float old, float src) {  // lane i <- lane i - 1
-      return __int_as_float(
-          __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x138, 0xf, 0xf, false));
-    };
-    auto shl = [](float old, float src) {  // lane i <- lane i + 1
-      return __int_as_float(
-          __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf, false));
-    };
-    v[i][0] = shr(lw.x, c.z);
-    v[i][1] = shr(lw.y, c.w);
-    v[i][2] = c.x;
-    v[i][3] = c.y;
-    v[i][4] = c.z;
-    v[i][5] = c.w;
-    v[i][6] = shl(rw.x, c.x);
-    v[i][7] = shl(rw.y, c.y);
-  }
-  // flat bits: fl[fr] bit fc is the cell (y0 - 1 + fr, x - 1 + fc), an
-  // interior cell equal to the minimum of its 3x3 window with NaN
-  // neighbours counting as -INF. Compared as order-preserving integer keys
-  // (NaN as -INF, -0 as +0, since the float comparison finds them equal),
-  // a non-NaN cell is flat exactly when the 3x3 minimum key (the cell
-  // included) is its own; the minimum is taken over columns, then rows.
-  unsigned kk[NR][8];
-#pragma unroll
-  for (int i = 0; i < NR; i++)
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const float q = v[i][j];
-      const unsigned u = __float_as_uint(q + 0.0f);  // -0 + 0 = +0
-      kk[i][j] = q != q ? 0x007fffffu : (u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u));
-    }
   // interior columns of the six flat columns (bit fc: x - 1 + fc)
   unsigned colin = 0u;
 #pragma unroll
   for (int fc = 0; fc < 6; fc++) colin |= (unsigned)(x - 1 + fc > 0 && x - 1 + fc < nx - 1) << fc;
-  unsigned fl[kRowsF + 2];
+  auto shr = [](float old, float src) {  // lane i <- lane i - 1
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x138, 0xf, 0xf, false));
+  };
+  auto shl = [](float old, float src) {  // lane i <- lane i + 1
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf, false));
+  };
+  float V[4][8];     // values of rows y - 3 .. y (V[3] newest)
+  unsigned K[3][8];  // keys of rows y - 2 .. y
+  unsigned F[3];     // flat bits of rows y - 3 .. y - 1 (bit fc: column x - 1 + fc)
 #pragma unroll
-  for (int fr = 0; fr < kRowsF + 2; fr++) {
-    const long long gy = y0 - 1 + fr;
-    unsigned cm[8];
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 8; j++) cm[j] = min(min(kk[fr][j], kk[fr + 1][j]), kk[fr + 2][j]);
-    unsigned bits = 0u;
+    for (int j = 0; j < 8; j++) V[i][j] = nan;
 #pragma unroll
-    for (int fc = 0; fc < 6; fc++) {
-      const float h = v[fr + 1][fc + 1];
-      const unsigned m9 = min(min(cm[fc], cm[fc + 1]), cm[fc + 2]);
-      bits |= (unsigned)(h == h && m9 == kk[fr + 1][fc + 1]) << fc;
-    }
-    fl[fr] = (gy > 0 && gy < ny - 1) ? (bits & colin) : 0u;
+  for (int i = 0; i < 3; i++) {
+    F[i] = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; j++) K[i][j] = 0x007fffffu;
   }
-  // flags of the own cells. A neighbour outside the grid is NaN in the
-  // window, never equal to anything, so it needs no test of its own.
-  // Neighbour n of a cell (n = 0..7: rows -1, 0, +1, columns -1, 0, +1 in
-  // order, the cell itself skipped): eq bit n if its value equals the
-  // cell's, nf bit n if it is flat.
   unsigned long long flats = 0;
+  const bool active = ys < ny;
+  constexpr int T = kStripF + 4;  // rows ys - 2 .. ys + kStripF + 1
+  static_assert(T % kAheadF == 0, "whole batches of rows");
+  RawRowF q[kAheadF];
 #pragma unroll
-  for (int r = 0; r < kRowsF; r++) {
-    const long long gy = y0 + r;
-    if (!active || !xin || gy >= ny) continue;
-    int o[4];
+  for (int d = 0; d < kAheadF; d++)
+    q[d] = load_row_f(dem, nx, active ? ny : 0, x, ys - 2 + d, lane, xin);
+  for (int t0 = 0; t0 < T; t0 += kAheadF) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const float h = v[r + 2][k + 2];
-      const unsigned flat = (fl[r + 1] >> (k + 1)) & 1u;
-      const unsigned nf = ((fl[r] >> k) & 7u) | (((fl[r + 1] >> k) & 1u) << 3) |
-                          (((fl[r + 1] >> (k + 2)) & 1u) << 4) | (((fl[r + 2] >> k) & 7u) << 5);
-      unsigned eq = 0u;
-      int n = 0;
+    for (int d = 0; d < kAheadF; d++) {
+      const int t = t0 + d;
+      const long long y = ys - 2 + t;
+      const RawRowF cur = q[d];
+      if (t + kAheadF < T) q[d] = load_row_f(dem, nx, active ? ny : 0, x, y + kAheadF, lane, xin);
+      float v[8];
+      v[0] = shr(cur.lw.x, cur.c.z);
+      v[1] = shr(cur.lw.y, cur.c.w);
+      v[2] = cur.c.x;
+      v[3] = cur.c.y;
+      v[4] = cur.c.z;
+      v[5] = cur.c.w;
+      v[6] = shl(cur.rw.x, cur.c.x);
+      v[7] = shl(cur.rw.y, cur.c.y);
 #pragma unroll
-      for (int dr = -1; dr <= 1; dr++)
+      for (int j = 0; j < 8; j++) {
+        V[0][j] = V[1][j];
+        V[1][j] = V[2][j];
+        V[2][j] = V[3][j];
+        V[3][j] = v[j];
+        K[0][j] = K[1][j];
+        K[1][j] = K[2][j];
+        K[2][j] = flat_key(v[j]);
+      }
+      // flat bits of row y - 1: an interior cell equal to the minimum of
+      // its 3x3 window, NaN neighbours counting as -INF; as keys, a non-NaN
+      // cell is flat exactly when the window's minimum key is its own
+      unsigned fnew = 0u;
+      {
+        unsigned cm[8];
 #pragma unroll
-        for (int dc = -1; dc <= 1; dc++) {
-          if (dr == 0 && dc == 0) continue;
-          eq |= (unsigned)(v[r + 2 + dr][k + 2 + dc] == h) << n;
-          n++;
+        for (int j = 0; j < 8; j++) cm[j] = min(min(K[0][j], K[1][j]), K[2][j]);
+#pragma unroll
+        for (int fc = 0; fc < 6; fc++) {
+          const float h = V[2][fc + 1];
+          const unsigned m9 = min(min(cm[fc], cm[fc + 1]), cm[fc + 2]);
+          fnew |= (unsigned)(h == h && m9 == K[1][fc + 1]) << fc;
         }
-      // sill: not flat, touches an equal flat; presill: flat, touches an
-      // equal non-flat cell (a sill)
-      const unsigned sill = !flat && (eq & nf) != 0u;
-      const unsigned presill = flat && (eq & ~nf) != 0u;
-      o[k] = (int)(flat | (sill << 1) | (presill << 2));
+        const long long gy = y - 1;
+        fnew = (gy > 0 && gy < ny - 1) ? (fnew & colin) : 0u;
+      }
+      F[0] = F[1];
+      F[1] = F[2];
+      F[2] = fnew;
+      // flags of row y - 2 (own cells). A neighbour outside the grid is NaN
+      // in the window, never equal to anything. Neighbour n of a cell
+      // (n = 0..7: rows -1, 0, +1, columns -1, 0, +1 in order, the cell
+      // itself skipped): eq bit n if its value equals the cell's, nf bit n
+      // if it is flat.
+      const long long gy = y - 2;
+      if (active && xin && gy >= ys && gy < ys + kStripF && gy < ny) {
+        int o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float h = V[1][k + 2];
+          const unsigned flat = (F[1] >> (k + 1)) & 1u;
+          const unsigned nf = ((F[0] >> k) & 7u) | (((F[1] >> k) & 1u) << 3) |
+                              (((F[1] >> (k + 2)) & 1u) << 4) | (((F[2] >> k) & 7u) << 5);
+          unsigned eq = 0u;
+          int n = 0;
+#pragma unroll
+          for (int dr = 0; dr < 3; dr++)
+#pragma unroll
+            for (int dc = -1; dc <= 1; dc++) {
+              if (dr == 1 && dc == 0) continue;
+              eq |= (unsigned)(V[dr][k + 2 + dc] == h) << n;
+              n++;
+            }
+          // sill: not flat, touches an equal flat; presill: flat, touches an
+          // equal non-flat cell (a sill)
+          const unsigned sill = !flat && (eq & nf) != 0u;
+          const unsigned presill = flat && (eq & ~nf) != 0u;
+          o[k] = (int)(flat | (sill << 1) | (presill << 2));
+        }
+        flats += __popc((F[1] >> 1) & 15u);
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const v4i ov = {o[0], o[1], o[2], o[3]};
+        __builtin_nontemporal_store(ov, reinterpret_cast<v4i*>(out + gy * nx + x));
+      }
     }
-    flats += __popc((fl[r + 1] >> 1) & 15u);
-    typedef int v4i __attribute__((ext_vector_type(4)));
-    const v4i ov = {o[0], o[1], o[2], o[3]};
-    __builtin_nontemporal_store(ov, reinterpret_cast<v4i*>(out + gy * nx + x));
   }
   if (partial) {
     for (int off = 32; off > 0; off >>= 1) flats += __shfl_xor(flats, off);
@@ -639,7 +683,7 @@ hipError_t run_identifyflats(int32_t* out, const float* dem, int64_t nx, int64_t
     return hipSuccess;
   }
   const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  const long long sgx = (nx + 255) / 256, sgy = (ny + 4 * kRowsF - 1) / (4 * kRowsF);
+  const long long sgx = (nx + 255) / 256, sgy = (ny + 4 * kStripF - 1) / (4 * kStripF);
   const bool stream_path = (nx & 3) == 0 && al(out) && al(dem) && sgy <= 65535;
   int grid = 1;
   hipError_t e = hipSuccess;
