@@ -549,7 +549,7 @@ __global__ __launch_bounds__(256) void gw_final(const float* original, const flo
   }
 }
 
-// ---- tie components, over their sorted keys (nk of them) ----
+// ---- tie components, over their keys (nk of them, in append order) ----
 
 #define GW_FOR_KEYS(n) \
   for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < (n); q += gridDim.x * blockDim.x)
@@ -686,7 +686,7 @@ __device__ unsigned find_seq(unsigned* P, unsigned x, unsigned n, unsigned* err)
 // of each set, both by key position.
 __device__ void replay_full(const Tie& t, const Grid& g, const unsigned long long* ev,
                             unsigned k, unsigned e, const float* original, const float* filled,
-                            unsigned* Mf, unsigned* err) {
+                            unsigned* Fc, unsigned* err) {
   const unsigned c = key_comp(ev[k], t.cb);
   for (unsigned q = k; q < e; q++) {
     const unsigned p = key_pixel(ev[q], t.cb);
@@ -714,13 +714,13 @@ __device__ void replay_full(const Tie& t, const Grid& g, const unsigned long lon
     }
   }
   const unsigned qc = key_pos(t, c, err);
-  if (qc != kNone) Mf[c] = t.Hs[find_seq(t.F, qc, t.nk, err)];
+  if (qc != kNone) Fc[c] = t.Hs[find_seq(t.F, qc, t.nk, err)];
 }
 
 // One thread per tie component: the replay of its scan steps (the nev
 // flagged keys, in scan order).
 __global__ void gw_tie_replay(Tie t, Grid g, const unsigned long long* ev, const unsigned* nev,
-                              const unsigned* K, const unsigned* W, unsigned* Mf,
+                              const unsigned* K, const unsigned* W, unsigned* Fc,
                               const float* original, const float* filled, unsigned* err) {
   const unsigned n = *nev;
   GW_FOR_KEYS(n) {
@@ -730,7 +730,7 @@ __global__ void gw_tie_replay(Tie t, Grid g, const unsigned long long* ev, const
     while (e < n && key_comp(ev[e], t.cb) == c) e++;
     const unsigned kind = K[c];
     if (kind == kMixed) {
-      replay_full(t, g, ev, q, e, original, filled, Mf, err);
+      replay_full(t, g, ev, q, e, original, filled, Fc, err);
       continue;
     }
     const unsigned wbits = W[c];
@@ -779,17 +779,17 @@ __global__ void gw_tie_replay(Tie t, Grid g, const unsigned long long* ev, const
     }
     if (!ok) continue;
     const unsigned qc = key_pos(t, c, err);
-    if (qc != kNone) Mf[c] = t.Hs[find_seq(t.F, qc, t.nk, err)];
+    if (qc != kNone) Fc[c] = t.Hs[find_seq(t.F, qc, t.nk, err)];
   }
 }
 
 // Outputs of the tie components' pixels, from their replayed labels.
-__global__ void gw_tie_outputs(Tie t, const unsigned* Mf, unsigned n, const float* original,
+__global__ void gw_tie_outputs(Tie t, const unsigned* Fc, unsigned n, const float* original,
                                const float* filled, float* costs, int64_t* conncomps,
                                unsigned* err) {
   GW_FOR_KEYS(t.nk) {
     const unsigned long long key = t.keys[q];
-    const unsigned p = key_pixel(key, t.cb), label = Mf[key_comp(key, t.cb)];
+    const unsigned p = key_pixel(key, t.cb), label = Fc[key_comp(key, t.cb)];
     if (label >= n) {
       atomicOr(err, 1u);
       continue;
