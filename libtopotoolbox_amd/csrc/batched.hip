@@ -186,6 +186,28 @@ __device__ void scatter_tile(unsigned long long* acc, const int* spill, const fl
     if (keys[k] >= 0) atomicAdd(&acc[keys[k]], vals[k]);
 }
 
+// The spill relaxation's visit as a call of its own. Inlined next to the
+// levels' visit, the two made the kernel spill to scratch (their registers
+// are allocated together); each alone does not. The call takes its LDS and
+// global pointers back into their address spaces, so that its accesses
+// stay LDS and global instructions rather than flat ones.
+template <class G>
+__device__ __attribute__((noinline)) unsigned batch_spill_visit(FillView v, long long x0,
+                                                                long long y0, bool first,
+                                                                unsigned nvis, unsigned ml,
+                                                                Sync* sink,
+                                                                FillLds<G, kModeSpill>& lds) {
+  using LdsT = FillLds<G, kModeSpill>;
+  typedef __attribute__((address_space(3))) char lds_char;
+  typedef __attribute__((address_space(1))) char glb_char;
+  LdsT& l = *reinterpret_cast<LdsT*>((char*)(lds_char*)(char*)&lds);
+  v.state = reinterpret_cast<float*>((char*)(glb_char*)(char*)v.state);
+  v.spill = reinterpret_cast<int*>((char*)(glb_char*)(char*)v.spill);
+  v.src = reinterpret_cast<const float*>((char*)(glb_char*)(char*)v.src);
+  v.bc = reinterpret_cast<const uint8_t*>((char*)(glb_char*)(char*)v.bc);
+  return fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, ml, sink, l);
+}
+
 template <class G, int OP>
 __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
   __shared__ BatchLds<G, OP> lds;
@@ -226,7 +248,7 @@ __global__ __launch_bounds__(G::THREADS) void batch_persistent(BatchArgs a) {
       if (ph == 2) {
         if (spill_skip(a.dirty, t, first, &lds.fill.bits)) return 0u;
         const unsigned bits =
-            fill_visit<G, kModeSpill>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.fill);
+            batch_spill_visit<G>(v, x0, y0, first, nvis, ml, a.q.stats_sink(), lds.fill);
         spill_mark(a.dirty, topo, t, bits);
         return bits;
       }
